@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Benchmark of the single-device skeleton path on one MI355X.
+
+Headline workload (one "step"): DM<float> 8192 x 8192, values uniform in
+[0,255) from a seed; mean-blur mapStencil at radius 1, 5 and 10 for 20
+iterations each (ping-pong, graph-captured), plus map(x*1.5f+3.f),
+zip(a+b), mapIndex(row*cols+col) and the compensated fold(sum).
+``value`` is the mean-blur stencil throughput in algorithmic GB/s
+(8 bytes per cell per iteration: one read, one write) with the grids
+resident in HBM.  Sub-results cover the other skeletons and workloads.
+
+Prints ONE JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s (1e9 B/s)
+ROWS = COLS = 8192
+RADII = (1, 5, 10)
+BLUR_ITERS = 20
+SEED = 20240611
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part "
+              "of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from muesli4_amd import hip
+    from muesli4_amd.hip import DeviceArray, call
+
+    hip.init(0)
+    info = hip.device_info()
+    log(f"[bench] device {info}")
+    n = ROWS * COLS
+    a = DeviceArray((ROWS, COLS), np.float32)
+    b = DeviceArray((ROWS, COLS), np.float32)
+    c = DeviceArray((ROWS, COLS), np.float32)
+    d_res = DeviceArray(1, np.float64)
+    scale = float(np.float32(255.0 / 16777216.0))
+    inb = C.c_int()
+    ms = C.c_float()
+
+    def timed(fn, reps):
+        call("msl_timer_start")
+        for _ in range(reps):
+            fn()
+        call("msl_timer_stop", C.byref(ms))
+        return ms.value / reps
+
+    # ---- per-kernel timings (average launch duration on the library stream)
+    def fresh():
+        call("msl_fill_uniform_f32", a.ptr, n, SEED, scale)
+        call("msl_fill_uniform_f32", b.ptr, n, SEED + 1, scale)
+
+    fresh()
+    hip.synchronize()
+    reps = 20
+    kernel_ms = {}
+    kernel_ms["map"] = timed(lambda: call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0), reps)
+    kernel_ms["zip"] = timed(lambda: call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n), reps)
+    kernel_ms["mapindex"] = timed(lambda: call("msl_mapindex_lin_f32", c.ptr, ROWS, COLS), reps)
+    kernel_ms["fold"] = timed(lambda: call("msl_fold_sum_f32", a.ptr, n, d_res.ptr), reps)
+    for r in RADII:
+        kernel_ms[f"blur_r{r}"] = timed(lambda: call("msl_stencil2d_mean_f32", c.ptr, a.ptr, ROWS, COLS, r, 0.0), reps)
+    kernel_bytes = {"map": 8 * n, "zip": 12 * n, "mapindex": 4 * n, "fold": 4 * n,
+                    **{f"blur_r{r}": 8 * n for r in RADII}}
+    kernels = {k: {"ms": round(v, 5), "GBs": round(kernel_bytes[k] / (v * 1e-3) / 1e9, 1),
+                   "pct_hbm_peak": round(100 * kernel_bytes[k] / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
+               for k, v in kernel_ms.items()}
+    log(f"[bench] kernels {kernels}")
+
+    # ---- headline step ------------------------------------------------------
+    def step():
+        call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0)
+        call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n)
+        call("msl_mapindex_lin_f32", c.ptr, ROWS, COLS)
+        call("msl_fold_sum_f32", a.ptr, n, d_res.ptr)
+        for r in RADII:
+            call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, r, 0.0, BLUR_ITERS, C.byref(inb))
+
+    for _ in range(args.warmup):
+        fresh()
+        step()
+    hip.synchronize()
+    step_ms = []
+    blur_ms = []
+    for _ in range(args.steps):
+        fresh()
+        hip.synchronize()
+        t0 = time.perf_counter()
+        call("msl_timer_start")
+        call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0)
+        call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n)
+        call("msl_mapindex_lin_f32", c.ptr, ROWS, COLS)
+        call("msl_fold_sum_f32", a.ptr, n, d_res.ptr)
+        call("msl_timer_stop", C.byref(ms))
+        elem_ms = ms.value
+        call("msl_timer_start")
+        for r in RADII:
+            call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, r, 0.0, BLUR_ITERS, C.byref(inb))
+        call("msl_timer_stop", C.byref(ms))
+        hip.synchronize()
+        t1 = time.perf_counter()
+        blur_ms.append(ms.value)
+        step_ms.append((t1 - t0) * 1e3)
+    blur_bytes = 8 * n * BLUR_ITERS * len(RADII)
+    blur_t = float(np.median(blur_ms)) * 1e-3
+    value = blur_bytes / blur_t / 1e9
+    log(f"[bench] headline blur GB/s {value:.1f} step ms {np.median(step_ms):.3f}")
+
+    # ---- roofline of the dominant kernel (mean blur r=1) ---------------------
+    dom = "blur_r1"
+    achieved = kernel_bytes[dom] / (kernel_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_blur_r1_8192.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": "strip_stencil_kernel<MeanPolicy<1>> (8192x8192, 8 B/cell)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+    result = {
+        "metric": "stencil GB/s (% HBM peak)",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(np.median(step_ms)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (SplitMix64-seeded uniform [0,255))",
+        "config": {"workload": "DM<float> 8192x8192 mean-blur mapStencil r=1,5,10 x 20 iterations each "
+                               "(+ map x*1.5f+3.f, zip a+b, mapIndex row*cols+col, compensated fold sum)",
+                   "rows": ROWS, "cols": COLS, "radii": list(RADII), "iterations": BLUR_ITERS},
+        "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 1),
+        "roofline": roofline,
+        "kernels": kernels,
+        "device": info,
+    }
+
+    if not args.only_headline:
+        result["gol"] = bench_gol(hip, call, DeviceArray)
+        result["heat3d"] = bench_heat(hip, call, DeviceArray)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(value)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+def bench_gol(hip, call, DeviceArray):
+    rows = cols = 1024
+    iters = 100
+    n = rows * cols
+    a = DeviceArray((rows, cols), np.int32)
+    b = DeviceArray((rows, cols), np.int32)
+    inb = C.c_int()
+    ms = C.c_float()
+    thr = int(0.3 * 2.0 ** 64)
+    call("msl_fill_bernoulli_i32", a.ptr, n, SEED, thr)
+    call("msl_gol_iterate_i32", a.ptr, b.ptr, rows, cols, iters, C.byref(inb))  # capture + warm
+    times = []
+    for _ in range(5):
+        call("msl_fill_bernoulli_i32", a.ptr, n, SEED, thr)
+        call("msl_timer_start")
+        call("msl_gol_iterate_i32", a.ptr, b.ptr, rows, cols, iters, C.byref(inb))
+        s = C.c_int64()
+        fin = b if inb.value else a
+        call("msl_fold_sum_i32_host", fin.ptr, n, C.byref(s))
+        call("msl_timer_stop", C.byref(ms))
+        times.append(ms.value)
+    t = float(np.median(times)) * 1e-3
+    out = {"workload": "GoL DM<int> 1024x1024 B3/S23, 100 iterations + fold(sum)", "ms": round(t * 1e3, 3),
+           "cells_per_s": round(n * iters / t, 1), "effective_GBs": round(8 * n * iters / t / 1e9, 1),
+           "live_cells": int(s.value), "note": "8 MiB working set is cache-resident; % HBM not meaningful"}
+    log(f"[bench] gol {out}")
+    return out
+
+
+def bench_heat(hip, call, DeviceArray):
+    nx = ny = nz = 512
+    iters, every = 200, 10
+    n = nx * ny * nz
+    a = DeviceArray((nz, ny, nx), np.float32)
+    b = DeviceArray((nz, ny, nx), np.float32)
+    res = (C.c_float * (iters // every))()
+    inb = C.c_int()
+    ms = C.c_float()
+    unit = float(np.float32(1.0 / 16777216.0))
+    call("msl_fill_uniform_f32", a.ptr, n, SEED, unit)
+    call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+    times = []
+    for _ in range(3):
+        call("msl_fill_uniform_f32", a.ptr, n, SEED, unit)
+        call("msl_timer_start")
+        call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+        call("msl_timer_stop", C.byref(ms))
+        times.append(ms.value)
+    t = float(np.median(times)) * 1e-3
+    # algorithmic bytes: 8 B/cell/step (residual fused: read of the old value is the same read)
+    gbs = 8 * n * iters / t / 1e9
+    out = {"workload": "DC<float> 512^3 heat7 (sum of 6 neighbours / 6, nv=0), 200 iterations, "
+                       "fused max|delta| residual every 10, hipGraph loop",
+           "ms": round(t * 1e3, 3), "GBs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
+           "residuals": [float(x) for x in res]}
+    log(f"[bench] heat {out['GBs']} GB/s")
+    return out
+
+
+def cpu_baseline(gpu_value):
+    """Oracle (OpenMP C) on the host cores, same metric and mix as the GPU
+    headline: full 8192x8192 grid, one mean-blur iteration at each of r=1, 5,
+    10 per round, rounds repeated until about 10 s have elapsed (at most 3)."""
+    from oracle import refcpu as R
+    threads = R.num_threads()
+    n = ROWS * COLS
+    x = R.fill_uniform_f32(n, SEED).reshape(ROWS, COLS)
+    R.stencil2d_mean_f32(x[:16], 1, 0.0)  # warm the thread pool
+    per_t = {r: [] for r in RADII}
+    rounds = 0
+    total_t = 0.0
+    while rounds < 3 and (rounds == 0 or total_t < 10.0):
+        for r in RADII:
+            t0 = time.perf_counter()
+            R.stencil2d_mean_f32(x, r, 0.0)
+            dt = time.perf_counter() - t0
+            per_t[r].append(dt)
+            total_t += dt
+            log(f"[bench] cpu r={r} {dt:.2f}s")
+        rounds += 1
+    med = {r: float(np.median(v)) for r, v in per_t.items()}
+    v = 8 * n * len(RADII) / sum(med.values()) / 1e9
+    return {"value": round(v, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"oracle mean-blur on the full 8192x8192 grid, one iteration at each of r=1,5,10 per round, "
+                      f"{rounds} round(s), {total_t:.1f} s of CPU time; value = 3 x 8 B/cell / sum of per-radius "
+                      "median iteration times (the GPU headline's equal-iterations mix)",
+            "per_radius_GBs": {f"r{r}": round(8 * n / t / 1e9, 3) for r, t in med.items()},
+            "gpu_over_cpu": round(gpu_value / v, 1) if v > 0 else None}
+
+
+if __name__ == "__main__":
+    main()

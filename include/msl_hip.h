@@ -1,0 +1,163 @@
+/*
+ * msl_hip.h -- C ABI of the MI355X (gfx950) skeleton library libmsl_hip.so.
+ *
+ * This is the drop-in boundary under the msl::DA / DM / DC<T> class
+ * templates (muesli4_amd/include/).  The reference (NinaHerrmann/muesli4) is
+ * header-only C++/CUDA with no FFI; each entry point below names the
+ * reference skeleton or runtime function whose device work it replaces.
+ *
+ * Conventions
+ *  - Every function returns int: MSL_OK (0) or a negative MSL_E_* code.
+ *    msl_last_error() returns a message for the calling thread's last error.
+ *    Nothing throws across this boundary.  (Reference: errors print and
+ *    continue, src/muesli.cpp:292-294; fatal paths exit(), src/ds.cpp:231.)
+ *  - Pointers named d_* are device pointers (from msl_malloc or any HIP
+ *    allocation on the current device); h_* are host pointers.
+ *  - All launches are stream-ordered on the library stream (msl_stream()).
+ *    Entry points that return a value to the host synchronise that stream.
+ *  - x = column, y = row, z = depth.  DM layout row*cols+col, DC layout
+ *    (z*rows+y)*cols+x (reference include/nplmatrix.h:141-143,
+ *    include/plcube.h:121-123).
+ *  - Floating point: IEEE binary32, no FMA contraction, correctly rounded
+ *    division; elementwise and stencil results are bit-identical to a scalar
+ *    CPU loop with the same operation order.
+ */
+#ifndef MSL_HIP_H
+#define MSL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSL_OK 0
+#define MSL_E_INVALID (-1)   /* bad argument (shape, radius, null pointer)   */
+#define MSL_E_HIP (-2)       /* HIP runtime error                             */
+#define MSL_E_NOMEM (-3)     /* device allocation failed / budget exceeded    */
+#define MSL_E_STATE (-4)     /* not initialised, or call not allowed now      */
+#define MSL_E_TIMEOUT (-5)   /* bounded wait expired                          */
+#define MSL_E_FULL (-6)      /* queue full (serving back-pressure)            */
+
+/* ------------------------------------------------------------------ */
+/* Runtime (reference: msl::initSkeletons / terminateSkeletons /
+ * syncStreams, src/muesli.cpp:63-158, 213-223; one stream per device,
+ * src/muesli.cpp:78-85).                                              */
+int msl_init(int device);
+int msl_finalize(void);
+const char *msl_last_error(void);
+/* hipStream_t of the library stream, as void*. */
+void *msl_stream(void);
+int msl_synchronize(void);
+/* Device properties: compute units, XCD count estimate, LDS bytes per CU. */
+int msl_device_info(int *num_cus, int *warp_size, size_t *total_mem, char *name, int name_len);
+
+/* ------------------------------------------------------------------ */
+/* Residency (reference: DS constructors / updateDevice / updateHost /
+ * gather, src/ds.cpp:63-131, 441-484, 538-591).                        */
+int msl_malloc(void **d_ptr, size_t bytes);
+int msl_free(void *d_ptr);
+int msl_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int msl_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int msl_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+
+/* ------------------------------------------------------------------ */
+/* Fills (reference: DS::fill(const T&), src/ds.cpp:289-305; seeded
+ * generators replace the examples' unseeded rand(),
+ * examples/gameoflife.cpp:87-89).  Hash = SplitMix64(seed, index).   */
+int msl_fill_f32(float *d_out, size_t n, float value);
+int msl_fill_i32(int32_t *d_out, size_t n, int32_t value);
+/* d_out[i] = (float)(hash(seed,i) >> 40) * scale */
+int msl_fill_uniform_f32(float *d_out, size_t n, uint64_t seed, float scale);
+/* d_out[i] = hash(seed,i) < threshold ? 1 : 0 */
+int msl_fill_bernoulli_i32(int32_t *d_out, size_t n, uint64_t seed, uint64_t threshold);
+/* d_out[i] = (uint32)(hash(seed,i) >> 32) */
+int msl_fill_u32(uint32_t *d_out, size_t n, uint64_t seed);
+
+/* ------------------------------------------------------------------ */
+/* Elementwise skeletons.
+ * msl_map_affine_f32: d_out[i] = d_in[i]*a + b (two roundings).
+ *   Replaces DS::map / mapInPlace with functor x*a+b (src/ds.cpp:612-663,
+ *   kernel src/map_kernels.cu:36-43).  d_out == d_in is mapInPlace.     */
+int msl_map_affine_f32(float *d_out, const float *d_in, size_t n, float a, float b);
+/* uint32 wrapping variant (DA<int> map(x*a+b)). */
+int msl_map_affine_u32(uint32_t *d_out, const uint32_t *d_in, size_t n, uint32_t a, uint32_t b);
+/* d_out[i] = d_a[i] + d_b[i].  Replaces DS::zip / zipInPlace with a+b
+ * (src/ds.cpp:667-727, kernel src/zip_kernels.cu:37-43).               */
+int msl_zip_add_f32(float *d_out, const float *d_a, const float *d_b, size_t n);
+/* d_out[row*cols+col] = (float)(row*cols+col).  Replaces
+ * DM::mapIndexInPlace with f(row,col,v) = row*cols+col
+ * (src/dm.cpp:370-401, kernel src/map_kernels.cu:60-70).               */
+int msl_mapindex_lin_f32(float *d_out, int rows, int cols);
+
+/* ------------------------------------------------------------------ */
+/* Folds (reference: DS::fold, src/ds.cpp:739-912; kernel
+ * src/fold_kernels.cu:51-129).  Result stays on the device; the _host
+ * forms copy the scalar back (one small D2H, not the whole array).
+ * msl_fold_sum_f32: compensated FP32 (TwoSum per lane and across lanes),
+ *   FP64 only in the final cross-block combine.  d_result receives
+ *   {double sum} ; h_sum64 / h_sum32 receive the double and its float
+ *   rounding.  Deterministic for a given device.                        */
+int msl_fold_sum_f32(const float *d_in, size_t n, double *d_result);
+int msl_fold_sum_f32_host(const float *d_in, size_t n, double *h_sum64, float *h_sum32);
+/* Wrapping uint32 sum (int fold with defined overflow). */
+int msl_fold_sum_u32_host(const uint32_t *d_in, size_t n, uint32_t *h_sum);
+/* Exact int32 sum into int64 (GoL live cells). */
+int msl_fold_sum_i32_host(const int32_t *d_in, size_t n, int64_t *h_sum);
+/* max over d_in (n > 0). */
+int msl_fold_max_f32_host(const float *d_in, size_t n, float *h_max);
+
+/* ------------------------------------------------------------------ */
+/* 2-D stencils (reference: DM::mapStencil<f>, src/dm.cpp:1535-1591;
+ * view include/nplmatrix.h:123-139; kernel
+ * include/detail/map_kernels.cuh:159-168).  Neutral value nv outside.
+ * msl_stencil2d_mean_f32: out(x,y) = (sum_dy (sum_dx in(x+dx,y+dy)))/(2r+1)^2
+ *   with both sums starting at 0 and ascending; 1 <= r <= 16.            */
+int msl_stencil2d_mean_f32(float *d_out, const float *d_in, int rows, int cols, int r, float nv);
+/* iters ping-pong steps between d_a and d_b starting from d_a, captured
+ * once into a HIP graph and replayed.  *h_result_in_b = 1 when the final
+ * grid is in d_b (odd iters).                                           */
+int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, int r, float nv, int iters,
+                                   int *h_result_in_b);
+/* Game of Life B3/S23, cells outside are 0 (reference:
+ * examples/gameoflife.cpp:18-43 through DM::mapStencilMM,
+ * src/dm.cpp:615-1208).                                                 */
+int msl_gol_i32(int32_t *d_out, const int32_t *d_in, int rows, int cols);
+int msl_gol_iterate_i32(int32_t *d_a, int32_t *d_b, int rows, int cols, int iters, int *h_result_in_b);
+
+/* ------------------------------------------------------------------ */
+/* 3-D stencils (reference: DC::mapStencil<f>, src/dc.cpp:559-592; view
+ * include/plcube.h:105-119; kernel include/detail/map_kernels.cuh:140-149;
+ * functor heat3D examples/heatdiffusion.cpp:55-65).
+ * msl_heat7_f32: v = 0 + in(x-1) + in(x+1) + in(y-1) + in(y+1) + in(z-1)
+ *   + in(z+1) (left to right), out = v / 6.                              */
+int msl_heat7_f32(float *d_out, const float *d_in, int nx, int ny, int nz, float nv);
+/* heat7 step fused with max |out - in| folded into *d_maxdiff (atomic max;
+ * the caller zeroes it, e.g. with msl_fill_f32).                         */
+int msl_heat7_residual_f32(float *d_out, const float *d_in, int nx, int ny, int nz, float nv, float *d_maxdiff);
+/* iters steps ping-pong from d_a; every residual_every-th step (1-based)
+ * is the fused residual step and writes h_residuals[k] for k-th such step
+ * (h_residuals holds iters/residual_every floats; may be NULL when
+ * residual_every == 0).  Captured once into a HIP graph and replayed.   */
+int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float nv, int iters, int residual_every,
+                          float *h_residuals, int *h_result_in_b);
+
+/* ------------------------------------------------------------------ */
+/* Stream-ordered capture of any sequence of the calls above into a
+ * replayable HIP graph (the reference launches and synchronises every
+ * skeleton individually, src/muesli.cpp:213-223).                       */
+int msl_graph_begin(void);
+int msl_graph_end(void **graph);
+int msl_graph_launch(void *graph);
+int msl_graph_destroy(void *graph);
+
+/* ------------------------------------------------------------------ */
+/* Event timing on the library stream (for benchmarks). */
+int msl_timer_start(void);
+int msl_timer_stop(float *h_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+"""ctypes binding of libmsl_hip.so (the C ABI declared in include/msl_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libmsl_hip.so")
+
+MSL_OK = 0
+ERROR_NAMES = {-1: "MSL_E_INVALID", -2: "MSL_E_HIP", -3: "MSL_E_NOMEM", -4: "MSL_E_STATE",
+               -5: "MSL_E_TIMEOUT", -6: "MSL_E_FULL"}
+
+
+class MslError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, func: str, code: int, msg: str):
+        super().__init__(f"{func} -> {ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+        self.func = func
+
+
+p = C.c_void_p
+i32 = C.c_int
+u64 = C.c_uint64
+sz = C.c_size_t
+f32 = C.c_float
+PF = C.POINTER(C.c_float)
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+PU32 = C.POINTER(C.c_uint32)
+PI64 = C.POINTER(C.c_int64)
+PP = C.POINTER(C.c_void_p)
+
+# name -> argtypes (all return int unless listed in _RESTYPES)
+SIGNATURES = {
+    "msl_init": [i32],
+    "msl_finalize": [],
+    "msl_last_error": [],
+    "msl_stream": [],
+    "msl_synchronize": [],
+    "msl_device_info": [PI, PI, C.POINTER(C.c_size_t), C.c_char_p, i32],
+    "msl_malloc": [PP, sz],
+    "msl_free": [p],
+    "msl_memcpy_h2d": [p, p, sz],
+    "msl_memcpy_d2h": [p, p, sz],
+    "msl_memcpy_d2d": [p, p, sz],
+    "msl_fill_f32": [p, sz, f32],
+    "msl_fill_i32": [p, sz, C.c_int32],
+    "msl_fill_uniform_f32": [p, sz, u64, f32],
+    "msl_fill_bernoulli_i32": [p, sz, u64, u64],
+    "msl_fill_u32": [p, sz, u64],
+    "msl_map_affine_f32": [p, p, sz, f32, f32],
+    "msl_map_affine_u32": [p, p, sz, C.c_uint32, C.c_uint32],
+    "msl_zip_add_f32": [p, p, p, sz],
+    "msl_mapindex_lin_f32": [p, i32, i32],
+    "msl_fold_sum_f32": [p, sz, p],
+    "msl_fold_sum_f32_host": [p, sz, PD, PF],
+    "msl_fold_sum_u32_host": [p, sz, PU32],
+    "msl_fold_sum_i32_host": [p, sz, PI64],
+    "msl_fold_max_f32_host": [p, sz, PF],
+    "msl_stencil2d_mean_f32": [p, p, i32, i32, i32, f32],
+    "msl_stencil2d_mean_iterate_f32": [p, p, i32, i32, i32, f32, i32, PI],
+    "msl_gol_i32": [p, p, i32, i32],
+    "msl_gol_iterate_i32": [p, p, i32, i32, i32, PI],
+    "msl_heat7_f32": [p, p, i32, i32, i32, f32],
+    "msl_heat7_residual_f32": [p, p, i32, i32, i32, f32, p],
+    "msl_heat7_iterate_f32": [p, p, i32, i32, i32, f32, i32, i32, PF, PI],
+    "msl_graph_begin": [],
+    "msl_graph_end": [PP],
+    "msl_graph_launch": [p],
+    "msl_graph_destroy": [p],
+    "msl_timer_start": [],
+    "msl_timer_stop": [PF],
+}
+_RESTYPES = {"msl_last_error": C.c_char_p, "msl_stream": C.c_void_p}
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib() -> C.CDLL:
+    """Load libmsl_hip.so; raise loudly when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f"{_LIB_PATH} is missing: build it with `python -m muesli4_amd.build` "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(_LIB_PATH, mode=C.RTLD_GLOBAL)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = L
+    return L
+
+
+def call(name: str, *args) -> int:
+    rc = getattr(lib(), name)(*args)
+    if rc != MSL_OK:
+        msg = lib().msl_last_error()
+        raise MslError(name, rc, msg.decode() if msg else "")
+    return rc
+
+
+def init(device: int = 0) -> None:
+    call("msl_init", device)
+
+
+def synchronize() -> None:
+    call("msl_synchronize")
+
+
+def device_info() -> dict:
+    cus, ws, mem = C.c_int(), C.c_int(), C.c_size_t()
+    name = C.create_string_buffer(128)
+    call("msl_device_info", C.byref(cus), C.byref(ws), C.byref(mem), name, 128)
+    return {"num_cus": cus.value, "warp_size": ws.value, "total_mem": mem.value, "arch": name.value.decode()}
+
+
+class DeviceArray:
+    """A device buffer owned by Python (msl_malloc / msl_free)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * self.dtype.itemsize
+        ptr = C.c_void_p()
+        call("msl_malloc", C.byref(ptr), self.nbytes)
+        self.ptr = ptr.value
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.shape, arr.dtype)
+        if d.nbytes:
+            call("msl_memcpy_h2d", d.ptr, arr.ctypes.data, d.nbytes)
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("msl_memcpy_d2h", out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            call("msl_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and _lib is not None:
+                _lib.msl_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass

@@ -1,0 +1,360 @@
+"""GPU parity: every C-ABI skeleton against the oracle on the same seeded inputs.
+
+Bar: bit-exact for integer, elementwise-float and stencil results (the HIP
+kernels keep the oracle's operation order, no FMA contraction, IEEE
+division); the compensated float fold is checked against the oracle's FP64
+sum with a stated relative tolerance.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from muesli4_amd import hip
+from muesli4_amd.hip import DeviceArray, call
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+# Compensated FP32 fold vs FP64 oracle: relative error bound used in tests.
+FOLD_REL_TOL = 1e-6
+
+
+def note(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    hip.init(0)
+    yield
+    hip.synchronize()
+
+
+def dev_uniform(n, seed, scale=R.SCALE_255):
+    d = DeviceArray(n, np.float32)
+    call("msl_fill_uniform_f32", d.ptr, n, seed, float(scale))
+    return d
+
+
+# ---- fills --------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 3, 4, 4097, 1000003])
+def test_fills_bit_exact(n):
+    note(f"fills n={n}")
+    d = dev_uniform(n, 77)
+    np.testing.assert_array_equal(d.to_host(), R.fill_uniform_f32(n, 77))
+    b = DeviceArray(n, np.int32)
+    call("msl_fill_bernoulli_i32", b.ptr, n, 5, R.BERNOULLI_03)
+    np.testing.assert_array_equal(b.to_host(), R.fill_bernoulli_i32(n, 5))
+    u = DeviceArray(n, np.uint32)
+    call("msl_fill_u32", u.ptr, n, 6)
+    np.testing.assert_array_equal(u.to_host(), R.fill_u32(n, 6))
+    c = DeviceArray(n, np.float32)
+    call("msl_fill_f32", c.ptr, n, 2.5)
+    assert np.all(c.to_host() == np.float32(2.5))
+
+
+# ---- elementwise --------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 5, 4096, 65539, 8192 * 8192])
+def test_map_zip_bit_exact(n):
+    note(f"map/zip n={n}")
+    a = dev_uniform(n, 1)
+    b = dev_uniform(n, 2)
+    out = DeviceArray(n, np.float32)
+    ha, hb = R.fill_uniform_f32(n, 1), R.fill_uniform_f32(n, 2)
+    call("msl_map_affine_f32", out.ptr, a.ptr, n, 1.5, 3.0)
+    np.testing.assert_array_equal(out.to_host(), R.map_affine_f32(ha, 1.5, 3.0))
+    call("msl_zip_add_f32", out.ptr, a.ptr, b.ptr, n)
+    np.testing.assert_array_equal(out.to_host(), R.zip_add_f32(ha, hb))
+    # in place (mapInPlace)
+    call("msl_map_affine_f32", a.ptr, a.ptr, n, 1.5, 3.0)
+    np.testing.assert_array_equal(a.to_host(), R.map_affine_f32(ha, 1.5, 3.0))
+
+
+def test_map_unaligned_pointer():
+    note("map unaligned")
+    n = 1001
+    a = dev_uniform(n + 1, 3)
+    out = DeviceArray(n + 1, np.float32)
+    call("msl_map_affine_f32", out.ptr + 4, a.ptr + 4, n, 1.5, 3.0)
+    got = out.to_host()[1:]
+    np.testing.assert_array_equal(got, R.map_affine_f32(R.fill_uniform_f32(n + 1, 3)[1:], 1.5, 3.0))
+
+
+def test_map_u32_wraps():
+    note("map u32")
+    n = 100003
+    u = DeviceArray(n, np.uint32)
+    call("msl_fill_u32", u.ptr, n, 9)
+    out = DeviceArray(n, np.uint32)
+    call("msl_map_affine_u32", out.ptr, u.ptr, n, 3, 1)
+    np.testing.assert_array_equal(out.to_host(), R.map_affine_u32(R.fill_u32(n, 9), 3, 1))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 7), (37, 53), (8192, 8192)])
+def test_mapindex_bit_exact(shape):
+    note(f"mapindex {shape}")
+    rows, cols = shape
+    out = DeviceArray((rows, cols), np.float32)
+    call("msl_mapindex_lin_f32", out.ptr, rows, cols)
+    np.testing.assert_array_equal(out.to_host(), R.mapindex_lin_f32(rows, cols))
+
+
+# ---- folds --------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 7, 1000, 1 << 20, 8192 * 8192 + 3])
+def test_fold_sum_f32_vs_fp64(n):
+    note(f"fold f32 n={n}")
+    d = dev_uniform(n, 11)
+    s64, s32 = C.c_double(), C.c_float()
+    call("msl_fold_sum_f32_host", d.ptr, n, C.byref(s64), C.byref(s32))
+    ref = R.fold_sum_f64(R.fill_uniform_f32(n, 11))
+    rel = abs(s64.value - ref) / max(abs(ref), 1e-30)
+    assert rel <= FOLD_REL_TOL, (s64.value, ref, rel)
+    assert s32.value == np.float32(s64.value)
+    # deterministic
+    s64b = C.c_double()
+    call("msl_fold_sum_f32_host", d.ptr, n, C.byref(s64b), None)
+    assert s64b.value == s64.value
+
+
+def test_fold_sum_f32_exact_where_fp32_stagnates():
+    note("fold f32 ones")
+    n = (1 << 24) + 1000
+    d = DeviceArray(n, np.float32)
+    call("msl_fill_f32", d.ptr, n, 1.0)
+    s64 = C.c_double()
+    call("msl_fold_sum_f32_host", d.ptr, n, C.byref(s64), None)
+    assert s64.value == float(n)
+    assert R.fold_sum_f32_seq(np.ones(n, np.float32)) == 16777216.0  # the plain FP32 order stagnates
+
+
+def test_fold_sum_f32_device_result():
+    note("fold f32 device result")
+    n = 123457
+    d = dev_uniform(n, 12)
+    res = DeviceArray(1, np.float64)
+    call("msl_fold_sum_f32", d.ptr, n, res.ptr)
+    got = float(res.to_host()[0])
+    ref = R.fold_sum_f64(R.fill_uniform_f32(n, 12))
+    assert abs(got - ref) / ref <= FOLD_REL_TOL
+
+
+@pytest.mark.parametrize("n", [1, 4099, 10 ** 7 + 1])
+def test_int_folds_exact(n):
+    note(f"int folds n={n}")
+    u = DeviceArray(n, np.uint32)
+    call("msl_fill_u32", u.ptr, n, 21)
+    r = C.c_uint32()
+    call("msl_fold_sum_u32_host", u.ptr, n, C.byref(r))
+    assert r.value == R.fold_sum_u32(R.fill_u32(n, 21))
+    b = DeviceArray(n, np.int32)
+    call("msl_fill_bernoulli_i32", b.ptr, n, 22, R.BERNOULLI_03)
+    s = C.c_int64()
+    call("msl_fold_sum_i32_host", b.ptr, n, C.byref(s))
+    assert s.value == R.fold_sum_i32(R.fill_bernoulli_i32(n, 22))
+    f = dev_uniform(n, 23)
+    m = C.c_float()
+    call("msl_fold_max_f32_host", f.ptr, n, C.byref(m))
+    assert m.value == R.fold_max_f32(R.fill_uniform_f32(n, 23))
+
+
+def test_fold_max_rejects_empty():
+    note("fold max empty")
+    d = DeviceArray(4, np.float32)
+    m = C.c_float()
+    with pytest.raises(hip.MslError):
+        call("msl_fold_max_f32_host", d.ptr, 0, C.byref(m))
+
+
+# ---- 2-D stencils -------------------------------------------------------------
+SHAPES_2D = [(1, 1), (1, 300), (300, 1), (7, 9), (37, 53), (64, 256), (130, 1001), (256, 260)]
+
+
+@pytest.mark.parametrize("r", list(range(1, 11)))
+def test_mean_blur_bit_exact_small(r):
+    for (rows, cols) in SHAPES_2D:
+        for nv in (0.0, 3.5):
+            note(f"mean r={r} {rows}x{cols} nv={nv}")
+            n = rows * cols
+            a = dev_uniform(n, 100 + r)
+            out = DeviceArray((rows, cols), np.float32)
+            call("msl_stencil2d_mean_f32", out.ptr, a.ptr, rows, cols, r, nv)
+            ref = R.stencil2d_mean_f32(R.fill_uniform_f32(n, 100 + r).reshape(rows, cols), r, nv)
+            np.testing.assert_array_equal(out.to_host(), ref, err_msg=f"r={r} shape={rows}x{cols} nv={nv}")
+
+
+@pytest.mark.parametrize("r", [11, 13, 16])
+def test_mean_blur_large_radius(r):
+    rows, cols = 150, 333
+    note(f"mean r={r} {rows}x{cols}")
+    a = dev_uniform(rows * cols, r)
+    out = DeviceArray((rows, cols), np.float32)
+    call("msl_stencil2d_mean_f32", out.ptr, a.ptr, rows, cols, r, 1.0)
+    ref = R.stencil2d_mean_f32(R.fill_uniform_f32(rows * cols, r).reshape(rows, cols), r, 1.0)
+    np.testing.assert_array_equal(out.to_host(), ref)
+
+
+def test_mean_blur_rejects_bad_radius():
+    note("mean bad radius")
+    a = DeviceArray(16, np.float32)
+    b = DeviceArray(16, np.float32)
+    for r in (0, 17, -1):
+        with pytest.raises(hip.MslError):
+            call("msl_stencil2d_mean_f32", b.ptr, a.ptr, 4, 4, r, 0.0)
+    with pytest.raises(hip.MslError):
+        call("msl_stencil2d_mean_f32", a.ptr, a.ptr, 4, 4, 1, 0.0)
+
+
+@pytest.mark.parametrize("r", [1, 5, 10])
+def test_mean_blur_8192_single_step(r):
+    rows = cols = 8192 if r == 1 else 2048
+    note(f"mean full r={r} {rows}x{cols}")
+    n = rows * cols
+    a = dev_uniform(n, 5)
+    out = DeviceArray((rows, cols), np.float32)
+    call("msl_stencil2d_mean_f32", out.ptr, a.ptr, rows, cols, r, 0.0)
+    ref = R.stencil2d_mean_f32(R.fill_uniform_f32(n, 5).reshape(rows, cols), r, 0.0)
+    np.testing.assert_array_equal(out.to_host(), ref)
+
+
+@pytest.mark.parametrize("r,iters", [(1, 20), (5, 20), (10, 3)])
+def test_mean_blur_iterate_graph(r, iters):
+    rows, cols = 384, 520
+    note(f"mean iterate r={r} iters={iters}")
+    n = rows * cols
+    a = dev_uniform(n, 8)
+    b = DeviceArray((rows, cols), np.float32)
+    inb = C.c_int()
+    ref = R.fill_uniform_f32(n, 8).reshape(rows, cols)
+    for _ in range(iters):
+        ref = R.stencil2d_mean_f32(ref, r, 0.0)
+    for rep in range(2):  # second call replays the cached graph
+        call("msl_fill_uniform_f32", a.ptr, n, 8, float(R.SCALE_255))
+        call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, 0.0, iters, C.byref(inb))
+        got = (b if inb.value else a).to_host().reshape(rows, cols)
+        np.testing.assert_array_equal(got, ref, err_msg=f"replay {rep}")
+
+
+def test_gol_1024_100_iterations_and_fold():
+    rows = cols = 1024
+    iters = 100
+    note("gol 1024x1024 x100")
+    n = rows * cols
+    a = DeviceArray((rows, cols), np.int32)
+    b = DeviceArray((rows, cols), np.int32)
+    call("msl_fill_bernoulli_i32", a.ptr, n, 2024, R.BERNOULLI_03)
+    inb = C.c_int()
+    call("msl_gol_iterate_i32", a.ptr, b.ptr, rows, cols, iters, C.byref(inb))
+    fin = b if inb.value else a
+    ref = R.fill_bernoulli_i32(n, 2024).reshape(rows, cols)
+    for _ in range(iters):
+        ref = R.gol_i32(ref)
+    np.testing.assert_array_equal(fin.to_host(), ref)
+    s = C.c_int64()
+    call("msl_fold_sum_i32_host", fin.ptr, n, C.byref(s))
+    assert s.value == int(ref.sum())
+
+
+@pytest.mark.parametrize("shape", [(3, 5), (17, 33), (100, 260)])
+def test_gol_single_step_ragged(shape):
+    rows, cols = shape
+    note(f"gol {shape}")
+    n = rows * cols
+    a = DeviceArray((rows, cols), np.int32)
+    b = DeviceArray((rows, cols), np.int32)
+    call("msl_fill_bernoulli_i32", a.ptr, n, 31, R.BERNOULLI_03)
+    call("msl_gol_i32", b.ptr, a.ptr, rows, cols)
+    np.testing.assert_array_equal(b.to_host(), R.gol_i32(R.fill_bernoulli_i32(n, 31).reshape(rows, cols)))
+
+
+# ---- 3-D heat -------------------------------------------------------------------
+SHAPES_3D = [(1, 1, 1), (5, 7, 9), (17, 9, 65), (33, 70, 130), (64, 64, 64)]
+
+
+@pytest.mark.parametrize("shape", SHAPES_3D)
+def test_heat7_bit_exact(shape):
+    nz, ny, nx = shape
+    note(f"heat7 {shape}")
+    n = nx * ny * nz
+    a = dev_uniform(n, 40, R.SCALE_UNIT)
+    out = DeviceArray(shape, np.float32)
+    host = R.fill_uniform_f32(n, 40, R.SCALE_UNIT).reshape(shape)
+    for nv in (0.0, 0.25):
+        call("msl_heat7_f32", out.ptr, a.ptr, nx, ny, nz, nv)
+        np.testing.assert_array_equal(out.to_host(), R.heat7_f32(host, nv))
+    md = DeviceArray(1, np.float32)
+    call("msl_fill_f32", md.ptr, 1, 0.0)
+    call("msl_heat7_residual_f32", out.ptr, a.ptr, nx, ny, nz, 0.0, md.ptr)
+    ref, m = R.heat7_residual_f32(host, 0.0)
+    np.testing.assert_array_equal(out.to_host(), ref)
+    assert float(md.to_host()[0]) == m
+
+
+def test_heat7_512_single_step():
+    shape = (512, 512, 512)
+    note("heat7 512^3")
+    n = 512 ** 3
+    a = dev_uniform(n, 41, R.SCALE_UNIT)
+    out = DeviceArray(shape, np.float32)
+    call("msl_heat7_f32", out.ptr, a.ptr, 512, 512, 512, 0.0)
+    np.testing.assert_array_equal(out.to_host(), R.heat7_f32(R.fill_uniform_f32(n, 41, R.SCALE_UNIT).reshape(shape)))
+
+
+def test_heat7_iterate_with_residuals():
+    shape = (40, 48, 72)
+    nz, ny, nx = shape
+    iters, every = 30, 10
+    note("heat7 iterate")
+    n = nx * ny * nz
+    a = dev_uniform(n, 42, R.SCALE_UNIT)
+    b = DeviceArray(shape, np.float32)
+    res = (C.c_float * (iters // every))()
+    inb = C.c_int()
+    ref = R.fill_uniform_f32(n, 42, R.SCALE_UNIT).reshape(shape)
+    ref_res = []
+    for it in range(iters):
+        if (it + 1) % every == 0:
+            ref, m = R.heat7_residual_f32(ref, 0.0)
+            ref_res.append(m)
+        else:
+            ref = R.heat7_f32(ref, 0.0)
+    for rep in range(2):
+        call("msl_fill_uniform_f32", a.ptr, n, 42, float(R.SCALE_UNIT))
+        call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+        got = (b if inb.value else a).to_host()
+        np.testing.assert_array_equal(got, ref, err_msg=f"replay {rep}")
+        assert list(res) == ref_res
+
+
+# ---- graph capture of arbitrary calls ------------------------------------------
+def test_graph_capture_replays():
+    note("graph capture")
+    n = 100000
+    a = dev_uniform(n, 50)
+    out = DeviceArray(n, np.float32)
+    tmp = DeviceArray(n, np.float32)
+    g = C.c_void_p()
+    call("msl_graph_begin")
+    call("msl_map_affine_f32", tmp.ptr, a.ptr, n, 1.5, 3.0)
+    call("msl_zip_add_f32", out.ptr, tmp.ptr, a.ptr, n)
+    call("msl_graph_end", C.byref(g))
+    ha = R.fill_uniform_f32(n, 50)
+    expect = R.zip_add_f32(R.map_affine_f32(ha, 1.5, 3.0), ha)
+    for rep in range(4):
+        call("msl_fill_f32", out.ptr, n, -1.0)  # eager work between replays
+        call("msl_graph_launch", g)
+        np.testing.assert_array_equal(out.to_host(), expect, err_msg=f"replay {rep}")
+    call("msl_graph_destroy", g)
+
+
+def test_malloc_refused_during_capture():
+    note("malloc during capture")
+    call("msl_graph_begin")
+    ptr = C.c_void_p()
+    with pytest.raises(hip.MslError):
+        call("msl_malloc", C.byref(ptr), 1024)
+    g = C.c_void_p()
+    call("msl_graph_end", C.byref(g))
+    call("msl_graph_destroy", g)
