@@ -323,7 +323,7 @@ def test_heat7_iterate_with_residuals():
     for rep in range(2):
         call("msl_fill_uniform_f32", a.ptr, n, 42, float(R.SCALE_UNIT))
         call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
-        got = (b if inb.value else a).to_host()
+        got = (b if inb.value else a).to_host().reshape(shape)
         np.testing.assert_array_equal(got, ref, err_msg=f"replay {rep}")
         assert list(res) == ref_res
 
