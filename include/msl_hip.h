@@ -153,6 +153,13 @@ int msl_graph_launch(void *graph);
 int msl_graph_destroy(void *graph);
 
 /* ------------------------------------------------------------------ */
+/* Diagnostics (no reference counterpart).  The mean-blur kernels divide
+ * by (2r+1)^2 with a multiply/FMA sequence instead of the IEEE division
+ * instruction sequence; this runs both on all 2^32 float inputs on the
+ * device and returns the number of bitwise mismatches (0 expected).   */
+int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches);
+
+/* ------------------------------------------------------------------ */
 /* Event timing on the library stream (for benchmarks). */
 int msl_timer_start(void);
 int msl_timer_stop(float *h_ms);

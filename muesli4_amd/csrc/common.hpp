@@ -47,6 +47,9 @@ int check_launch(const char *what);
 // Scratch workspace owned by the context (grown outside graph capture only).
 int workspace(size_t bytes, void **ptr);
 
+// Integer tuning knob from the environment (read once per name), else dflt.
+int tune_int(const char *name, int dflt);
+
 // Number of workgroups that fills the device for a streaming kernel.
 inline int streaming_blocks(size_t work_items, int threads, int per_cu) {
     size_t want = (work_items + threads - 1) / threads;
@@ -67,6 +70,38 @@ __host__ __device__ inline uint64_t hash64(uint64_t seed, uint64_t idx) {
 // IEEE single operations that the compiler may not contract into an FMA.
 __device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
 __device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every outstanding global load of the wave, which would drain the
+// software prefetch queues of the stencil marches at each step; the
+// "local"-only fences make the barrier wait for LDS operations alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Orders one wave's own LDS accesses (RAW/WAR between its lanes).  A wave's
+// LDS instructions execute in issue order, so this is a compiler-level
+// barrier only: no wait on outstanding global loads, no workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Correctly rounded x / d for the divisors d = (2r+1)^2, r = 1..16, without
+// the IEEE division sequence: q0 = x*rcp (rcp = RN(1/d)), the remainder
+// x - q0*d is exact with one FMA, and one FMA correction gives RN(x/d).
+// When the remainder is 0 (q0 exact; covers -0) or NaN (x infinite or NaN)
+// q0 itself is the answer.  Checked bit-for-bit against IEEE division for
+// all 2^32 inputs of every such d by msl_selftest_div_odd_square.
+__device__ __forceinline__ float div_odd_square(float x, float d, float rcp) {
+    float q0 = __fmul_rn(x, rcp);
+    float r = __fmaf_rn(-q0, d, x);
+    float q1 = __fmaf_rn(r, rcp, q0);
+    return (r != 0.0f && r == r) ? q1 : q0;
+}
 
 // Remap blockIdx so that each XCD (blocks are dealt round-robin to the 8
 // XCDs) receives a contiguous range of logical tiles; neighbouring tiles

@@ -16,9 +16,12 @@ typedef int32_t v4i __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kUnroll = 4;
 
+// Plain loads and non-temporal stores: measured fastest on MI355X for
+// streams (the non-temporal store keeps the output from evicting the input
+// from the 256 MiB Infinity Cache; non-temporal loads were slower).
 template <typename V>
 __device__ __forceinline__ V ldg(const V *p) {
-    return __builtin_nontemporal_load(p);
+    return *p;
 }
 template <typename V>
 __device__ __forceinline__ void stg(V *p, V v) {
@@ -175,7 +178,7 @@ int launch_gen(S *out, size_t n, Op op, const char *name) {
     if (rc) return rc;
     if (n == 0) return MSL_OK;
     MSL_REQUIRE(out != nullptr, std::string(name) + ": null pointer");
-    int blocks = streaming_blocks((n + 3) / 4, kThreads, 8);
+    int blocks = streaming_blocks((n + 3) / 4, kThreads, tune_int("MSL_STREAM_BLOCKS_PER_CU", 1));
     if (aligned16(out))
         gen_kernel<V, S, Op><<<blocks, kThreads, 0, ctx().stream>>>(out, n, op);
     else
@@ -189,8 +192,8 @@ int launch_map(S *out, const S *in, size_t n, Op op, const char *name) {
     if (rc) return rc;
     if (n == 0) return MSL_OK;
     MSL_REQUIRE(out && in, std::string(name) + ": null pointer");
-    // 4 vectors in flight per thread, 8 blocks of 256 per CU.
-    int blocks = streaming_blocks((n + 15) / 16, kThreads, 8);
+    // 4 vectors in flight per thread; one 256-thread block per CU by default.
+    int blocks = streaming_blocks((n + 15) / 16, kThreads, tune_int("MSL_STREAM_BLOCKS_PER_CU", 1));
     if (aligned16(out) && aligned16(in))
         stream1_kernel<V, S, Op><<<blocks, kThreads, 0, ctx().stream>>>(out, in, n, op);
     else
@@ -204,7 +207,7 @@ int launch_zip(S *out, const S *a, const S *b, size_t n, Op op, const char *name
     if (rc) return rc;
     if (n == 0) return MSL_OK;
     MSL_REQUIRE(out && a && b, std::string(name) + ": null pointer");
-    int blocks = streaming_blocks((n + 15) / 16, kThreads, 8);
+    int blocks = streaming_blocks((n + 15) / 16, kThreads, tune_int("MSL_STREAM_BLOCKS_PER_CU", 1));
     if (aligned16(out) && aligned16(a) && aligned16(b))
         stream2_kernel<V, S, Op><<<blocks, kThreads, 0, ctx().stream>>>(out, a, b, n, op);
     else
@@ -212,29 +215,13 @@ int launch_zip(S *out, const S *a, const S *b, size_t n, Op op, const char *name
     return check_launch(name);
 }
 
-// mapIndex over a rows x cols matrix: each thread produces 4 consecutive
-// columns of one row; row and column come from a 2-D decomposition of the
-// vector index without a per-element division.
-__global__ __launch_bounds__(kThreads) void mapindex_lin_kernel(float *__restrict__ out, int rows, int cols) {
-    // grid: x over column quads, y over rows (grid-stride in y).
-    const int q = blockIdx.x * kThreads + threadIdx.x;
-    const int c0 = q * 4;
-    if (c0 >= cols) return;
-    for (int row = blockIdx.y; row < rows; row += gridDim.y) {
-        float *dst = out + (size_t)row * cols;
-        const int base = row * cols;
-        if (c0 + 3 < cols && ((((size_t)row * cols) & 3) == 0)) {
-            v4f v;
-            v.x = (float)(base + c0 + 0);
-            v.y = (float)(base + c0 + 1);
-            v.z = (float)(base + c0 + 2);
-            v.w = (float)(base + c0 + 3);
-            stg(reinterpret_cast<v4f *>(dst + c0), v);
-        } else {
-            for (int c = c0; c < cols && c < c0 + 4; c++) dst[c] = (float)(base + c);
-        }
-    }
-}
+// mapIndex with f(row, col, v) = row*cols + col over a row-major matrix:
+// for every flat index k = row*cols + col (row = k / cols, col = k % cols)
+// the int value row*cols + col equals k, so the element is (float)k - the
+// same int->float conversion (round to nearest even) the functor performs.
+struct LinIndexF32 {
+    __device__ float operator()(size_t k) const { return (float)(int32_t)k; }
+};
 
 }  // namespace mslhip
 
@@ -275,14 +262,7 @@ int msl_mapindex_lin_f32(float *d_out, int rows, int cols) {
     MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX, "msl_mapindex_lin_f32: rows*cols overflows int");
     if (rows == 0 || cols == 0) return MSL_OK;
     MSL_REQUIRE(d_out != nullptr, "msl_mapindex_lin_f32: null pointer");
-    int quads = (cols + 3) / 4;
-    int gx = (quads + kThreads - 1) / kThreads;
-    int want_rows = (ctx().num_cus * 8 + gx - 1) / gx;
-    int gy = rows < want_rows ? rows : want_rows;
-    if (gy > 65535) gy = 65535;
-    dim3 grid(gx, gy);
-    mapindex_lin_kernel<<<grid, kThreads, 0, ctx().stream>>>(d_out, rows, cols);
-    return check_launch("msl_mapindex_lin_f32");
+    return launch_gen<v4f>(d_out, (size_t)rows * cols, LinIndexF32{}, "msl_mapindex_lin_f32");
 }
 
 }  // extern "C"

@@ -117,18 +117,38 @@ __global__ __launch_bounds__(kFoldThreads) void fold_stage1(const typename P::In
     if (vec_ok) {
         const size_t n4 = n / 4;
         const V *in4 = reinterpret_cast<const V *>(in);
-        for (; i + 3 * stride < n4; i += 4 * stride) {
-            V v0 = __builtin_nontemporal_load(in4 + i);
-            V v1 = __builtin_nontemporal_load(in4 + i + stride);
-            V v2 = __builtin_nontemporal_load(in4 + i + 2 * stride);
-            V v3 = __builtin_nontemporal_load(in4 + i + 3 * stride);
-            P::push(acc, v0.x); P::push(acc, v0.y); P::push(acc, v0.z); P::push(acc, v0.w);
-            P::push(acc, v1.x); P::push(acc, v1.y); P::push(acc, v1.z); P::push(acc, v1.w);
-            P::push(acc, v2.x); P::push(acc, v2.y); P::push(acc, v2.z); P::push(acc, v2.w);
-            P::push(acc, v3.x); P::push(acc, v3.y); P::push(acc, v3.z); P::push(acc, v3.w);
+        // Software pipeline: the next U vectors are requested before the
+        // current U are accumulated, so each lane keeps up to 2U 16-byte
+        // loads in flight.
+        constexpr int U = 4;
+        if (i + (U - 1) * stride < n4) {
+            V v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = in4[i + u * stride];
+            i += U * stride;
+            for (; i + (U - 1) * stride < n4; i += U * stride) {
+                V nv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) nv[u] = in4[i + u * stride];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    P::push(acc, v[u].x);
+                    P::push(acc, v[u].y);
+                    P::push(acc, v[u].z);
+                    P::push(acc, v[u].w);
+                    v[u] = nv[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                P::push(acc, v[u].x);
+                P::push(acc, v[u].y);
+                P::push(acc, v[u].z);
+                P::push(acc, v[u].w);
+            }
         }
         for (; i < n4; i += stride) {
-            V v = __builtin_nontemporal_load(in4 + i);
+            V v = in4[i];
             P::push(acc, v.x); P::push(acc, v.y); P::push(acc, v.z); P::push(acc, v.w);
         }
         done = n4 * 4;
@@ -172,7 +192,7 @@ __global__ __launch_bounds__(kFoldThreads) void fold_generic_stage2(const typena
 
 inline int fold_blocks(size_t n) {
     size_t want = (n + (size_t)kFoldThreads * 16 - 1) / ((size_t)kFoldThreads * 16);
-    size_t cap = (size_t)ctx().num_cus * 4;
+    size_t cap = (size_t)ctx().num_cus * tune_int("MSL_FOLD_BLOCKS_PER_CU", 2);
     if (want > cap) want = cap;
     if (want < 1) want = 1;
     return (int)want;

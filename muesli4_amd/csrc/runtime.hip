@@ -5,7 +5,9 @@
 // device, one stream; skeletons are stream-ordered instead of synchronising
 // after every call.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -38,6 +40,22 @@ Context &ctx() { return g_ctx; }
 int ensure_ready() {
     if (g_ctx.ready) return MSL_OK;
     return msl_init(0);
+}
+
+int tune_int(const char *name, int dflt) {
+    static std::mutex mu;
+    static std::map<std::string, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(name);
+    if (it != cache.end()) return it->second;
+    int v = dflt;
+    if (const char *e = std::getenv(name)) {
+        char *end = nullptr;
+        long x = std::strtol(e, &end, 10);
+        if (end != e && x > 0 && x < (1 << 20)) v = (int)x;
+    }
+    cache[name] = v;
+    return v;
 }
 
 int check_launch(const char *what) {

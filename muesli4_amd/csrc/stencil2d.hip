@@ -14,7 +14,8 @@
 //   3. row sums live in a register ring of 2R+1 entries (the loop is unrolled
 //      over the ring period so every ring index is static) and each output is
 //      the ordered sum of its 2R+1 row sums (dy ascending from 0.0f), divided
-//      by (2R+1)^2 with IEEE division.
+//      by (2R+1)^2, correctly rounded (div_odd_square: multiply plus one FMA
+//      correction, verified bit-identical to IEEE division on all inputs).
 // This keeps the exact operation order of the scalar definition (bit-exact)
 // while doing 2(2R+1) adds per output instead of (2R+1)^2, reading each
 // input element once from HBM (halo rows/columns come from L2) and writing
@@ -71,7 +72,8 @@ struct MeanPolicy {
     static constexpr int R = R_;
     static constexpr bool kCenter = false;
     typedef float T;
-    float denom;
+    float denom;  // (2R+1)^2
+    float rcp;    // RN(1 / denom)
     // Row sum of window w (index RP-R+c+d holds column x0+c+d-R).
     template <int RP>
     __device__ __forceinline__ float rowsum(const float *w, int c) const {
@@ -80,7 +82,10 @@ struct MeanPolicy {
         for (int d = 0; d <= 2 * R; d++) s = __fadd_rn(s, w[RP - R + c + d]);
         return s;
     }
-    __device__ __forceinline__ float finish(float colsum, float /*center*/) const { return __fdiv_rn(colsum, denom); }
+    // Correctly rounded colsum / denom (identical to IEEE division).
+    __device__ __forceinline__ float finish(float colsum, float /*center*/) const {
+        return div_odd_square(colsum, denom, rcp);
+    }
     __device__ __forceinline__ float colsum_init() const { return 0.0f; }
     __device__ __forceinline__ float acc(float a, float b) const { return __fadd_rn(a, b); }
 };
@@ -111,39 +116,45 @@ struct PrefetchDepth {
     static constexpr int value = R <= 2 ? 8 : (R <= 5 ? 4 : 2);
 };
 
-// One wave per (strip, segment) item; 64-thread workgroups so that the LDS
-// hand-off between lanes is ordered by __syncthreads() on a single wave.
-template <typename Pol>
-__global__ __launch_bounds__(64) void strip_stencil_kernel(typename Pol::T *__restrict__ out,
-                                                           const typename Pol::T *__restrict__ in, int rows, int cols,
-                                                           typename Pol::T nv, int seg, int strips_x, int nitems,
-                                                           int vec, Pol pol) {
-    typedef typename Pol::T T;
-    constexpr int R = Pol::R;
-    constexpr int RP = (R + 3) / 4 * 4;      // halo padded to whole quads
-    constexpr int LDSW = kStripW + 2 * RP;   // floats per staged row
-    constexpr int NQ = (2 * RP) / 4 + 1;     // 16-byte window reads per lane
-    constexpr int PER = 2 * R + 1;           // ring period
-    constexpr int D = PrefetchDepth<R>::value;
-    constexpr int U = PER * D;               // unroll: multiple of PER, D and 2
-    __shared__ __attribute__((aligned(16))) T lds[2][LDSW];
+template <int R>
+struct StripConsts {
+    static constexpr int RP = (R + 3) / 4 * 4;     // halo padded to whole quads
+    static constexpr int LDSW = kStripW + 2 * RP;  // elements per staged row
+    static constexpr int NQ = (2 * RP) / 4 + 1;    // 16-byte window reads per lane
+    static constexpr int PER = 2 * R + 1;          // ring period
+    static constexpr int D = PrefetchDepth<R>::value;
+    static constexpr int U = PER * D;              // unroll: multiple of PER, D and 2
+};
 
-    const int lane = threadIdx.x;
-    const int item = (int)xcd_tile(blockIdx.x, gridDim.x);
-    if (item >= nitems) return;
-    const int sx = item % strips_x;
-    const int sy = item / strips_x;
-    const int xs = sx * kStripW;
+// March one wave down its (strip, segment).  INTERIOR: the strip plus halo
+// columns, the segment plus halo and prefetch rows, all lie inside the grid
+// and rows are 16-byte aligned, so loads and stores need no bounds checks;
+// otherwise every access is checked and reads outside the grid give nv.
+template <typename Pol, bool INTERIOR>
+__device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
+                                            int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
+                                            bool vecb, const Pol &pol,
+                                            typename Pol::T (*lds)[StripConsts<Pol::R>::LDSW]) {
+    typedef typename Pol::T T;
+    typedef typename NativeVec<T>::type V;
+    constexpr int R = Pol::R;
+    constexpr int RP = StripConsts<R>::RP;
+    constexpr int NQ = StripConsts<R>::NQ;
+    constexpr int PER = StripConsts<R>::PER;
+    constexpr int D = StripConsts<R>::D;
+    constexpr int U = StripConsts<R>::U;
+
+    const int lane = threadIdx.x & 63;
     const int x0 = xs + 4 * lane;
-    const int y0 = sy * seg;
-    const int yend = min(y0 + seg, rows);
     const int total = (yend - y0) + 2 * R;  // input rows y0-R .. yend-1+R
-    // halo quad owned by this lane: left quads for lanes [0, RP/4), right
-    // quads for lanes [RP/4, RP/2); column of its first element.
+    // Halo quads: left quads for lanes [0, RP/4), right quads for lanes
+    // [RP/4, RP/2).  In the interior path every other lane re-loads and
+    // re-stages its own centre quad instead, so no lane diverges.
     const bool has_halo = lane < RP / 2;
     const int hx = lane < RP / 4 ? xs - RP + 4 * lane : xs + kStripW + 4 * (lane - RP / 4);
     const int hslot = lane < RP / 4 ? 4 * lane : RP + kStripW + 4 * (lane - RP / 4);
-    const bool vecb = vec != 0;
+    const int hx_i = has_halo ? hx : x0;
+    const int hslot_i = has_halo ? hslot : RP + 4 * lane;
 
     Vec4<T> pc[D], ph[D];  // prefetched centre / halo quads
     Vec4<T> ring[PER];     // row sums
@@ -151,7 +162,13 @@ __global__ __launch_bounds__(64) void strip_stencil_kernel(typename Pol::T *__re
 
     auto fetch = [&](int t, Vec4<T> &c, Vec4<T> &h) {
         const int yy = y0 - R + t;
-        if (t < total && yy >= 0 && yy < rows) {
+        if (INTERIOR) {
+            const T *rowp = in + (size_t)yy * cols;
+            V a = *reinterpret_cast<const V *>(rowp + x0);
+            V b = *reinterpret_cast<const V *>(rowp + hx_i);
+            c.v[0] = a.x; c.v[1] = a.y; c.v[2] = a.z; c.v[3] = a.w;
+            h.v[0] = b.x; h.v[1] = b.y; h.v[2] = b.z; h.v[3] = b.w;
+        } else if (t < total && yy >= 0 && yy < rows) {
             const T *rowp = in + (size_t)yy * cols;
             c = load_quad<T>(rowp, x0, cols, nv, vecb);
             if (has_halo) h = load_quad<T>(rowp, hx, cols, nv, vecb);
@@ -164,82 +181,139 @@ __global__ __launch_bounds__(64) void strip_stencil_kernel(typename Pol::T *__re
         }
     };
 
+    // One row step.  k_slot = t % D (prefetch slot), k_ring = t % PER,
+    // k_buf = t & 1; after unrolling these are compile-time constants, so
+    // every register array index is static.  'store' emits output row t-2R.
+    auto step = [&](int t, int k_slot, int k_ring, int k_buf, bool store) {
+        T *buf = lds[k_buf];
+        // stage row t
+        *reinterpret_cast<V *>(buf + RP + 4 * lane) =
+            V{pc[k_slot].v[0], pc[k_slot].v[1], pc[k_slot].v[2], pc[k_slot].v[3]};
+        if (INTERIOR) {
+            *reinterpret_cast<V *>(buf + hslot_i) =
+                V{ph[k_slot].v[0], ph[k_slot].v[1], ph[k_slot].v[2], ph[k_slot].v[3]};
+        } else if (has_halo) {
+            *reinterpret_cast<V *>(buf + hslot) =
+                V{ph[k_slot].v[0], ph[k_slot].v[1], ph[k_slot].v[2], ph[k_slot].v[3]};
+        }
+        // prefetch row t + D into the slot just consumed
+        fetch(t + D, pc[k_slot], ph[k_slot]);
+        wave_lds_sync();
+        T w[4 * NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            V v = *reinterpret_cast<const V *>(buf + 4 * lane + 4 * q);
+            w[4 * q + 0] = v.x;
+            w[4 * q + 1] = v.y;
+            w[4 * q + 2] = v.z;
+            w[4 * q + 3] = v.w;
+        }
+        Vec4<T> rs;
+#pragma unroll
+        for (int c = 0; c < 4; c++) rs.v[c] = pol.template rowsum<RP>(w, c);
+        ring[k_ring] = rs;
+        if constexpr (Pol::kCenter) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) cring[k_ring].v[c] = w[RP + c];
+        }
+        if (store) {
+            const int y = y0 - 2 * R + t;
+            Vec4<T> o;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                T s = pol.colsum_init();
+#pragma unroll
+                for (int k = 0; k < PER; k++) s = pol.acc(s, ring[(k_ring + 1 + k) % PER].v[c]);
+                T centre = T(0);
+                if constexpr (Pol::kCenter) centre = cring[(k_ring + 1 + R) % PER].v[c];
+                o.v[c] = pol.finish(s, centre);
+            }
+            T *orow = out + (size_t)y * cols;
+            if (INTERIOR || (vecb && x0 + 3 < cols)) {
+                __builtin_nontemporal_store(V{o.v[0], o.v[1], o.v[2], o.v[3]}, reinterpret_cast<V *>(orow + x0));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (x0 + c < cols) orow[x0 + c] = o.v[c];
+            }
+        }
+    };
+
 #pragma unroll
     for (int d = 0; d < D; d++) fetch(d, pc[d], ph[d]);
 
-    for (int t0 = 0; t0 < total; t0 += U) {
+    if constexpr (INTERIOR) {
+        // Branch-free schedule (the segment length is a multiple of U): 2R
+        // prologue rows without output, then whole unrolled blocks of U rows.
+        // Keeping conditionals out of the unrolled body lets the compiler's
+        // memory-counter waits track each prefetched row exactly instead of
+        // draining the prefetch queue at every step.
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = t0 + u;
-            if (t < total) {
-                T *buf = lds[u & 1];
-                // stage row t
-                *reinterpret_cast<typename NativeVec<T>::type *>(buf + RP + 4 * lane) =
-                    typename NativeVec<T>::type{pc[u % D].v[0], pc[u % D].v[1], pc[u % D].v[2], pc[u % D].v[3]};
-                if (has_halo)
-                    *reinterpret_cast<typename NativeVec<T>::type *>(buf + hslot) =
-                        typename NativeVec<T>::type{ph[u % D].v[0], ph[u % D].v[1], ph[u % D].v[2], ph[u % D].v[3]};
-                // prefetch row t + D into the slot just consumed
-                fetch(t + D, pc[u % D], ph[u % D]);
-                __syncthreads();
-                T w[4 * NQ];
+        for (int t = 0; t < 2 * R; t++) step(t, t % D, t % PER, t & 1, false);
+        for (int tb = 2 * R; tb < total; tb += U) {
 #pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    typename NativeVec<T>::type v =
-                        *reinterpret_cast<const typename NativeVec<T>::type *>(buf + 4 * lane + 4 * q);
-                    w[4 * q + 0] = v.x;
-                    w[4 * q + 1] = v.y;
-                    w[4 * q + 2] = v.z;
-                    w[4 * q + 3] = v.w;
-                }
-                Vec4<T> rs;
+            for (int u = 0; u < U; u++)
+                step(tb + u, (2 * R + u) % D, (2 * R + u) % PER, (2 * R + u) & 1, true);
+        }
+    } else {
+        for (int t0 = 0; t0 < total; t0 += U) {
 #pragma unroll
-                for (int c = 0; c < 4; c++) rs.v[c] = pol.template rowsum<RP>(w, c);
-                ring[u % PER] = rs;
-                if constexpr (Pol::kCenter) {
-#pragma unroll
-                    for (int c = 0; c < 4; c++) cring[u % PER].v[c] = w[RP + c];
-                }
-                if (t >= 2 * R) {
-                    const int y = y0 - R + t - R;
-                    Vec4<T> o;
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        T s = pol.colsum_init();
-#pragma unroll
-                        for (int k = 0; k < PER; k++) s = pol.acc(s, ring[(u + 1 + k) % PER].v[c]);
-                        T centre = T(0);
-                        if constexpr (Pol::kCenter) centre = cring[(u + 1 + R) % PER].v[c];
-                        o.v[c] = pol.finish(s, centre);
-                    }
-                    T *orow = out + (size_t)y * cols;
-                    if (vecb && x0 + 3 < cols) {
-                        __builtin_nontemporal_store(typename NativeVec<T>::type{o.v[0], o.v[1], o.v[2], o.v[3]},
-                                                    reinterpret_cast<typename NativeVec<T>::type *>(orow + x0));
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < 4; c++)
-                            if (x0 + c < cols) orow[x0 + c] = o.v[c];
-                    }
-                }
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u;
+                if (t < total) step(t, u % D, u % PER, u & 1, t >= 2 * R);
             }
         }
     }
+}
+
+// One wave per (strip, segment) item, four independent waves per 256-thread
+// workgroup; each wave stages rows in its own LDS region, so only
+// wave-level ordering is needed (no workgroup barrier).
+template <typename Pol>
+__global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__restrict__ out,
+                                                            const typename Pol::T *__restrict__ in, int rows, int cols,
+                                                            typename Pol::T nv, int seg, int strips_x, int nitems,
+                                                            int vec, Pol pol) {
+    typedef typename Pol::T T;
+    constexpr int R = Pol::R;
+    constexpr int RP = StripConsts<R>::RP;
+    constexpr int D = StripConsts<R>::D;
+    __shared__ __attribute__((aligned(16))) T lds[4][2][StripConsts<R>::LDSW];
+
+    const int wave = threadIdx.x >> 6;
+    const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
+    if (item >= nitems) return;
+    const int sx = item % strips_x;
+    const int sy = item / strips_x;
+    const int xs = sx * kStripW;
+    const int y0 = sy * seg;
+    const int yend = min(y0 + seg, rows);
+    const bool interior = vec != 0 && xs - RP >= 0 && xs + kStripW + RP <= cols && y0 - R >= 0 &&
+                          (long)yend + R + D <= (long)rows && (yend - y0) % StripConsts<R>::U == 0;
+    if (interior)
+        strip_march<Pol, true>(out, in, rows, cols, nv, xs, y0, yend, true, pol, lds[wave]);
+    else
+        strip_march<Pol, false>(out, in, rows, cols, nv, xs, y0, yend, vec != 0, pol, lds[wave]);
 }
 
 struct StripShape {
     int seg, strips_x, segs, nitems;
 };
 
-inline StripShape strip_shape(int rows, int cols, int seg) {
+// Segment length: about 16 waves per CU, rounded to a whole number of
+// unrolled blocks (unroll) so that interior segments take the branch-free
+// schedule.
+inline StripShape strip_shape(int rows, int cols, int seg, int unroll) {
     StripShape s;
     s.strips_x = (cols + kStripW - 1) / kStripW;
+    if (seg <= 0) seg = tune_int("MSL_STRIP_SEG", 0);
     if (seg <= 0) {
-        // Aim for >= 16 waves per CU without very short segments.
         long target = (long)ctx().num_cus * 16;
         long segs = (target + s.strips_x - 1) / s.strips_x;
         seg = (int)((rows + segs - 1) / segs);
         if (seg < 32) seg = 32;
+        seg = (seg + unroll / 2) / unroll * unroll;
+        if (seg < unroll) seg = unroll;
     }
     s.seg = seg;
     s.segs = (rows + seg - 1) / seg;
@@ -250,9 +324,9 @@ inline StripShape strip_shape(int rows, int cols, int seg) {
 template <typename Pol>
 int launch_strip(typename Pol::T *out, const typename Pol::T *in, int rows, int cols, typename Pol::T nv, Pol pol,
                  int seg, const char *name) {
-    StripShape s = strip_shape(rows, cols, seg);
+    StripShape s = strip_shape(rows, cols, seg, StripConsts<Pol::R>::U);
     int vec = ((cols & 3) == 0) && ((((uintptr_t)in) & 15u) == 0) && ((((uintptr_t)out) & 15u) == 0);
-    strip_stencil_kernel<Pol><<<s.nitems, 64, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg, s.strips_x,
+    strip_stencil_kernel<Pol><<<(s.nitems + 3) / 4, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg, s.strips_x,
                                                                    s.nitems, vec, pol);
     return check_launch(name);
 }
@@ -261,6 +335,7 @@ template <int R>
 int launch_mean_r(float *out, const float *in, int rows, int cols, float nv, int seg) {
     MeanPolicy<R> pol;
     pol.denom = (float)((2 * R + 1) * (2 * R + 1));
+    pol.rcp = 1.0f / pol.denom;
     return launch_strip(out, in, rows, cols, nv, pol, seg, "msl_stencil2d_mean_f32");
 }
 
@@ -293,6 +368,31 @@ int gol_step(int32_t *out, const int32_t *in, int rows, int cols) {
     LifePolicy pol;
     pol.denom = 1.0f;
     return launch_strip(out, in, rows, cols, (int32_t)0, pol, 0, "msl_gol_i32");
+}
+
+// ---- exhaustive check of div_odd_square against IEEE division -------------
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+
+// Every 32-bit pattern x: compare div_odd_square(x) with x / d bit for bit
+// (NaN against NaN counts as equal); one atomic per block.
+__global__ __launch_bounds__(256) void div_check_kernel(float d, float rcp, unsigned long long *count) {
+    __shared__ unsigned int part[256];
+    unsigned int bad = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32); i += stride) {
+        const float x = __uint_as_float((unsigned int)i);
+        const float ref = __fdiv_rn(x, d);
+        const float got = div_odd_square(x, d, rcp);
+        const bool same = (__float_as_uint(ref) == __float_as_uint(got)) || (ref != ref && got != got);
+        bad += same ? 0u : 1u;
+    }
+    part[threadIdx.x] = bad;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part[0]) atomicAdd(count, (unsigned long long)part[0]);
 }
 
 // ---- graph cache for iteration loops ---------------------------------------
@@ -373,6 +473,27 @@ int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, i
         }
         return MSL_OK;
     });
+}
+
+int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    MSL_REQUIRE(r >= 1 && r <= kMaxMeanR, "msl_selftest_div_odd_square: r must be in [1, 16]");
+    MSL_REQUIRE(h_mismatches != nullptr, "msl_selftest_div_odd_square: null out-pointer");
+    unsigned long long *d_count = nullptr;
+    MSL_HIP_TRY(hipMalloc(&d_count, sizeof(unsigned long long)));
+    zero_u64_kernel<<<1, 1, 0, ctx().stream>>>(d_count);
+    const float d = (float)((2 * r + 1) * (2 * r + 1));
+    div_check_kernel<<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(d, 1.0f / d, d_count);
+    rc = check_launch("msl_selftest_div_odd_square");
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, ctx().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
+    hipFree(d_count);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "msl_selftest_div_odd_square");
+    *h_mismatches = h;
+    return MSL_OK;
 }
 
 int msl_gol_i32(int32_t *d_out, const int32_t *d_in, int rows, int cols) { return gol_step(d_out, d_in, rows, cols); }

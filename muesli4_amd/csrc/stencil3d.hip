@@ -25,13 +25,20 @@ namespace mslhip {
 
 constexpr int kTX = 64;  // x extent of a tile (one wave row)
 constexpr int kWY = 4;   // waves per workgroup (y)
-constexpr int kRPT = 2;  // y rows per thread
+constexpr int kRPT = 4;  // y rows per thread
 constexpr int kTY = kWY * kRPT;
+constexpr int kP = 8;    // plane ring: z-1 .. z+6 (planes z+2 .. z+6 in flight)
 
+// Branch-free cell read: the address is clamped into the cube and the value
+// replaced by nv when the cell lies outside, so loads are never predicated
+// and the compiler's memory-counter waits stay exact across the march.
 __device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, int y, int z, int nx, int ny, int nz,
                                          float nv) {
-    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return nv;
-    return in[((size_t)z * ny + y) * nx + x];
+    const int xc = min(max(x, 0), nx - 1);
+    const int yc = min(max(y, 0), ny - 1);
+    const int zc = min(max(z, 0), nz - 1);
+    const float v = in[((size_t)zc * ny + yc) * nx + xc];
+    return (x == xc && y == yc && z == zc) ? v : nv;
 }
 
 template <bool kResidual>
@@ -54,71 +61,74 @@ __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, con
 
     // Halo duty for plane loads: wave 0 loads row ybase-1, wave 1 row
     // ybase+kTY, wave 2 lanes 0..kTY-1 the column x=-1 side, lanes
-    // 32..32+kTY-1 the x=+kTX side.
-    int hx = 0, hy = 0, hsx = 0, hsy = 0;
+    // 32..32+kTY-1 the x=+kTX side.  Other lanes load a harmless duplicate
+    // of their own cell and skip the LDS write.
+    int hx = x, hy = ybase + ty, hsx = 0, hsy = 0;
     bool hal = false;
     if (ty == 0) {
-        hal = true; hx = x; hy = ybase - 1; hsx = tx + 1; hsy = 0;
+        hal = true; hy = ybase - 1; hsx = tx + 1; hsy = 0;
     } else if (ty == 1) {
-        hal = true; hx = x; hy = ybase + kTY; hsx = tx + 1; hsy = kTY + 1;
+        hal = true; hy = ybase + kTY; hsx = tx + 1; hsy = kTY + 1;
     } else if (ty == 2 && tx < kTY) {
         hal = true; hx = xt * kTX - 1; hy = ybase + tx; hsx = 0; hsy = tx + 1;
     } else if (ty == 2 && tx >= 32 && tx < 32 + kTY) {
         hal = true; hx = xt * kTX + kTX; hy = ybase + (tx - 32); hsx = kTX + 1; hsy = tx - 32 + 1;
     }
 
-    float below[kRPT], cur[kRPT], above[kRPT];
-    float hcur = nv, habove = nv;
+    // Ring of planes per cell: slot(zz) = (zz - z0 + 1) & (kP-1).
+    float q[kRPT][kP];
+    float hq[kP];
 #pragma unroll
-    for (int r = 0; r < kRPT; r++) {
-        const int y = ybase + ty + r * kWY;
-        below[r] = ld_cell(in, x, y, z0 - 1, nx, ny, nz, nv);
-        cur[r] = ld_cell(in, x, y, z0, nx, ny, nz, nv);
-        above[r] = ld_cell(in, x, y, z0 + 1, nx, ny, nz, nv);
-    }
-    if (hal) {
-        hcur = ld_cell(in, hx, hy, z0, nx, ny, nz, nv);
-        habove = ld_cell(in, hx, hy, z0 + 1, nx, ny, nz, nv);
+    for (int k = 0; k < kP - 1; k++) {
+        const int zz = z0 - 1 + k;
+#pragma unroll
+        for (int r = 0; r < kRPT; r++) q[r][k] = ld_cell(in, x, ybase + ty + r * kWY, zz, nx, ny, nz, nv);
+        hq[k] = ld_cell(in, hx, hy, zz, nx, ny, nz, nv);
     }
     float dmax = 0.0f;
-    int b = 0;
-    for (int z = z0; z < z1; z++, b ^= 1) {
-        // stage plane z
+    // Whole blocks of kP planes; planes past z1 are computed but not stored.
+    for (int zb = z0; zb < z1; zb += kP) {
 #pragma unroll
-        for (int r = 0; r < kRPT; r++) tile[b][ty + r * kWY + 1][tx + 1] = cur[r];
-        if (hal) tile[b][hsy][hsx] = hcur;
-        // prefetch plane z+2 (becomes 'above' next step)
-        float nxt[kRPT];
-        float hnxt = nv;
+        for (int u = 0; u < kP; u++) {
+            const int z = zb + u;
+            const int sb = u & (kP - 1);        // z-1
+            const int sc = (u + 1) & (kP - 1);  // z
+            const int sa = (u + 2) & (kP - 1);  // z+1
+            float (*tl)[kTX + 2] = tile[u & 1];
+            // stage plane z
 #pragma unroll
-        for (int r = 0; r < kRPT; r++) nxt[r] = ld_cell(in, x, ybase + ty + r * kWY, z + 2, nx, ny, nz, nv);
-        if (hal) hnxt = ld_cell(in, hx, hy, z + 2, nx, ny, nz, nv);
-        __syncthreads();
+            for (int r = 0; r < kRPT; r++) tl[ty + r * kWY + 1][tx + 1] = q[r][sc];
+            if (hal) tl[hsy][hsx] = hq[sc];
+            // prefetch plane z+6 into the slot of plane z+6 (= slot of z-2)
+            {
+                const int sn = (u + 7) & (kP - 1);
+                const int zz = z + 6;
 #pragma unroll
-        for (int r = 0; r < kRPT; r++) {
-            const int ly = ty + r * kWY + 1;
-            const int y = ybase + ty + r * kWY;
-            float v = 0.0f;
-            v = __fadd_rn(v, tile[b][ly][tx]);      // x-1
-            v = __fadd_rn(v, tile[b][ly][tx + 2]);  // x+1
-            v = __fadd_rn(v, tile[b][ly - 1][tx + 1]);  // y-1
-            v = __fadd_rn(v, tile[b][ly + 1][tx + 1]);  // y+1
-            v = __fadd_rn(v, below[r]);             // z-1
-            v = __fadd_rn(v, above[r]);             // z+1
-            v = __fdiv_rn(v, 6.0f);
-            if (x < nx && y < ny) {
-                __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
-                if (kResidual) {
-                    float d = fabsf(__fsub_rn(v, cur[r]));
-                    dmax = d > dmax ? d : dmax;
+                for (int r = 0; r < kRPT; r++) q[r][sn] = ld_cell(in, x, ybase + ty + r * kWY, zz, nx, ny, nz, nv);
+                hq[sn] = ld_cell(in, hx, hy, zz, nx, ny, nz, nv);
+            }
+            lds_barrier();
+#pragma unroll
+            for (int r = 0; r < kRPT; r++) {
+                const int ly = ty + r * kWY + 1;
+                const int y = ybase + ty + r * kWY;
+                float v = 0.0f;
+                v = __fadd_rn(v, tl[ly][tx]);          // x-1
+                v = __fadd_rn(v, tl[ly][tx + 2]);      // x+1
+                v = __fadd_rn(v, tl[ly - 1][tx + 1]);  // y-1
+                v = __fadd_rn(v, tl[ly + 1][tx + 1]);  // y+1
+                v = __fadd_rn(v, q[r][sb]);            // z-1
+                v = __fadd_rn(v, q[r][sa]);            // z+1
+                v = __fdiv_rn(v, 6.0f);
+                if (z < z1 && x < nx && y < ny) {
+                    __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
+                    if (kResidual) {
+                        float d = fabsf(__fsub_rn(v, q[r][sc]));
+                        dmax = d > dmax ? d : dmax;
+                    }
                 }
             }
-            below[r] = cur[r];
-            cur[r] = above[r];
-            above[r] = nxt[r];
         }
-        hcur = habove;
-        habove = hnxt;
     }
     if (kResidual) {
 #pragma unroll
@@ -149,7 +159,9 @@ inline HeatShape heat_shape(int nx, int ny, int nz) {
     long zsegs = (target + cols - 1) / cols;
     if (zsegs < 1) zsegs = 1;
     int zseg = (int)((nz + zsegs - 1) / zsegs);
+    zseg = (zseg + kP - 1) / kP * kP;  // whole blocks of the plane ring
     if (zseg < 16) zseg = 16;
+    if (int z = tune_int("MSL_HEAT_ZSEG", 0)) zseg = z;
     s.zseg = zseg;
     s.zsegs = (nz + zseg - 1) / zseg;
     s.nitems = s.xtiles * s.ytiles * s.zsegs;

@@ -70,6 +70,7 @@ SIGNATURES = {
     "msl_heat7_f32": [p, p, i32, i32, i32, f32],
     "msl_heat7_residual_f32": [p, p, i32, i32, i32, f32, p],
     "msl_heat7_iterate_f32": [p, p, i32, i32, i32, f32, i32, i32, PF, PI],
+    "msl_selftest_div_odd_square": [i32, C.POINTER(C.c_uint64)],
     "msl_graph_begin": [],
     "msl_graph_end": [PP],
     "msl_graph_launch": [p],

@@ -185,6 +185,15 @@ def test_mean_blur_bit_exact_small(r):
             np.testing.assert_array_equal(out.to_host(), ref, err_msg=f"r={r} shape={rows}x{cols} nv={nv}")
 
 
+def test_fast_division_matches_ieee_on_all_inputs():
+    # every 32-bit pattern, every divisor (2r+1)^2 used by the blur kernels
+    for r in range(1, 17):
+        note(f"div selftest r={r}")
+        bad = C.c_uint64(1)
+        call("msl_selftest_div_odd_square", r, C.byref(bad))
+        assert bad.value == 0, (r, bad.value)
+
+
 @pytest.mark.parametrize("r", [11, 13, 16])
 def test_mean_blur_large_radius(r):
     rows, cols = 150, 333
