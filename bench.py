@@ -65,6 +65,8 @@ def main():
     ms = C.c_float()
 
     def timed(fn, reps):
+        for _ in range(3):  # first launches load the code object / grow the workspace
+            fn()
         call("msl_timer_start")
         for _ in range(reps):
             fn()

@@ -3,16 +3,16 @@
 // Reference: DS::fold (src/ds.cpp:739-912) copies the whole array to the host,
 // cudaMallocs a partial buffer and runs a shared-memory tree whose second pass
 // is skipped beyond 1024 blocks (src/fold_kernels.cu:298-403).  Here a fold is
-// two launches and no host copy: stage 1 streams the array once with 16-byte
+// two launches and no host copy: the array is streamed once with 16-byte
 // loads (each lane accumulates, then a wave64 shuffle tree and a 4-wave LDS
-// combine produce one partial per block); stage 2 is one block that combines
-// the partials in a fixed order.  Results stay in device memory.
+// combine produce one partial per block) and one block combines the partials
+// in a fixed order.  Results stay in device memory.
 //
 // Compensated sum: every lane keeps (s, c) and adds each element with Knuth's
 // TwoSum (s' = s + x exactly split into s' and the rounding error, which is
-// accumulated in c); partial pairs are merged with TwoSum as well.  Stage 2
-// converts each (s, c) to FP64 and adds them in block order - the only FP64
-// arithmetic in the fold.
+// accumulated in c); partial pairs are merged with TwoSum as well.  The final
+// combine converts each (s, c) to FP64 and adds them in block order - the
+// only FP64 arithmetic in the fold.
 #include "common.hpp"
 
 namespace mslhip {
@@ -105,40 +105,24 @@ __device__ __forceinline__ typename P::Part block_reduce(typename P::Part v) {
     return r;  // identical in every thread
 }
 
-// Stage 1: one partial per block.
-template <typename P>
-__global__ __launch_bounds__(kFoldThreads) void fold_stage1(const typename P::In *__restrict__ in, size_t n,
+// Stage 1: every block streams its grid-stride share with four 16-byte loads
+// in flight per lane and reduces it to one partial.  The order of every
+// combination is fixed, so results are deterministic for a given grid size.
+template <typename P, bool kF64Final>
+__global__ __launch_bounds__(kFoldThreads) void fold_kernel(const typename P::In *__restrict__ in, size_t n,
                                                             typename P::Part *__restrict__ partials, int vec_ok) {
     using V = typename P::V;
     typename P::Part acc = P::init();
     const size_t stride = (size_t)gridDim.x * kFoldThreads;
     size_t i = (size_t)blockIdx.x * kFoldThreads + threadIdx.x;
-    size_t done = 0;
     if (vec_ok) {
         const size_t n4 = n / 4;
         const V *in4 = reinterpret_cast<const V *>(in);
-        // Software pipeline: the next U vectors are requested before the
-        // current U are accumulated, so each lane keeps up to 2U 16-byte
-        // loads in flight.
         constexpr int U = 4;
-        if (i + (U - 1) * stride < n4) {
+        for (; i + (U - 1) * stride < n4; i += U * stride) {
             V v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) v[u] = in4[i + u * stride];
-            i += U * stride;
-            for (; i + (U - 1) * stride < n4; i += U * stride) {
-                V nv[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) nv[u] = in4[i + u * stride];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    P::push(acc, v[u].x);
-                    P::push(acc, v[u].y);
-                    P::push(acc, v[u].z);
-                    P::push(acc, v[u].w);
-                    v[u] = nv[u];
-                }
-            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 P::push(acc, v[u].x);
@@ -151,72 +135,76 @@ __global__ __launch_bounds__(kFoldThreads) void fold_stage1(const typename P::In
             V v = in4[i];
             P::push(acc, v.x); P::push(acc, v.y); P::push(acc, v.z); P::push(acc, v.w);
         }
-        done = n4 * 4;
-        i = done + (size_t)blockIdx.x * kFoldThreads + threadIdx.x;
+        i = n4 * 4 + (size_t)blockIdx.x * kFoldThreads + threadIdx.x;
     }
     for (; i < n; i += stride) P::push(acc, in[i]);
-    typename P::Part r = block_reduce<P>(acc);
+    const typename P::Part r = block_reduce<P>(acc);
+
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-// Stage 2 for the compensated sum: FP64 combine of (s, c) partials in a
-// fixed order (thread t takes partials t, t+256, ...; then a fixed tree).
-__global__ __launch_bounds__(kFoldThreads) void fold_sum_f32_stage2(const Pair *__restrict__ partials, int nparts,
-                                                                     double *__restrict__ result) {
-    double acc = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += kFoldThreads) {
-        Pair p = partials[k];
-        acc += (double)p.s;
-        acc += (double)p.c;
-    }
+// Final combine (second launch, one block): partials in block order.  A
+// second launch is used instead of a last-block-done counter because the
+// agent-scope release fence that counter needs writes back the whole L2 of
+// each XCD and costs more than the extra launch.
+template <typename P, bool kF64Final>
+__global__ __launch_bounds__(kFoldThreads) void fold_final_kernel(const typename P::Part *__restrict__ partials,
+                                                                  int nparts, void *__restrict__ result) {
+    if constexpr (kF64Final) {
+        // Compensated sum: FP64 combine of the (s, c) partials, the only FP64
+        // arithmetic of the fold (thread t takes partials t, t+256, ...).
+        double d = 0.0;
+        for (int k = threadIdx.x; k < nparts; k += kFoldThreads) {
+            const Pair p = partials[k];
+            d += (double)p.s;
+            d += (double)p.c;
+        }
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
-    __shared__ double lds[kFoldWaves];
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = lds[0];
-        for (int w = 1; w < kFoldWaves; w++) r += lds[w];
-        result[0] = r;
+        for (int m = 32; m >= 1; m >>= 1) d += __shfl_xor(d, m);
+        __shared__ double lds[kFoldWaves];
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = lds[0];
+            for (int w = 1; w < kFoldWaves; w++) t += lds[w];
+            *reinterpret_cast<double *>(result) = t;
+        }
+    } else {
+        typename P::Part a = P::init();
+        for (int k = threadIdx.x; k < nparts; k += kFoldThreads) a = P::combine(a, partials[k]);
+        const typename P::Part t = block_reduce<P>(a);
+        if (threadIdx.x == 0) *reinterpret_cast<typename P::Part *>(result) = t;
     }
-}
-
-template <typename P>
-__global__ __launch_bounds__(kFoldThreads) void fold_generic_stage2(const typename P::Part *__restrict__ partials,
-                                                                     int nparts, typename P::Part *__restrict__ result) {
-    typename P::Part acc = P::init();
-    for (int k = threadIdx.x; k < nparts; k += kFoldThreads) acc = P::combine(acc, partials[k]);
-    typename P::Part r = block_reduce<P>(acc);
-    if (threadIdx.x == 0) result[0] = r;
 }
 
 inline int fold_blocks(size_t n) {
     size_t want = (n + (size_t)kFoldThreads * 16 - 1) / ((size_t)kFoldThreads * 16);
-    size_t cap = (size_t)ctx().num_cus * tune_int("MSL_FOLD_BLOCKS_PER_CU", 2);
+    size_t cap = (size_t)ctx().num_cus * tune_int("MSL_FOLD_BLOCKS_PER_CU", 1);
     if (want > cap) want = cap;
     if (want < 1) want = 1;
     return (int)want;
 }
 
-// Workspace layout: [0, 256) result scalars, then partials.
+// Workspace layout: [0, 256) result scalars, partials from byte 512.
 constexpr size_t kResultBytes = 256;
+constexpr size_t kPartialsOffset = 512;
 
-template <typename P, typename Stage2>
-int run_fold(const typename P::In *in, size_t n, void *d_result, Stage2 stage2, const char *name) {
+template <typename P, bool kF64Final>
+int run_fold(const typename P::In *in, size_t n, void *d_result, const char *name) {
     int rc = ensure_ready();
     if (rc) return rc;
     MSL_REQUIRE(in != nullptr || n == 0, std::string(name) + ": null input");
-    int blocks = fold_blocks(n);
+    const int blocks = fold_blocks(n);
     void *ws = nullptr;
-    rc = workspace(kResultBytes + (size_t)blocks * sizeof(typename P::Part), &ws);
+    rc = workspace(kPartialsOffset + (size_t)blocks * sizeof(typename P::Part), &ws);
     if (rc) return rc;
-    auto *partials = reinterpret_cast<typename P::Part *>((char *)ws + kResultBytes);
+    auto *partials = reinterpret_cast<typename P::Part *>((char *)ws + kPartialsOffset);
     if (!d_result) d_result = ws;
-    int vec_ok = (((uintptr_t)in) & 15u) == 0;
-    fold_stage1<P><<<blocks, kFoldThreads, 0, ctx().stream>>>(in, n, partials, vec_ok);
+    const int vec_ok = (((uintptr_t)in) & 15u) == 0;
+    fold_kernel<P, kF64Final><<<blocks, kFoldThreads, 0, ctx().stream>>>(in, n, partials, vec_ok);
     rc = check_launch(name);
     if (rc) return rc;
-    stage2(partials, blocks, d_result);
+    fold_final_kernel<P, kF64Final><<<1, kFoldThreads, 0, ctx().stream>>>(partials, blocks, d_result);
     return check_launch(name);
 }
 
@@ -230,21 +218,11 @@ extern "C" {
 
 int msl_fold_sum_f32(const float *d_in, size_t n, double *d_result) {
     MSL_REQUIRE(d_result != nullptr, "msl_fold_sum_f32: null result pointer");
-    return run_fold<CompSumPolicy>(d_in, n, d_result,
-                                   [](Pair *parts, int np, void *res) {
-                                       fold_sum_f32_stage2<<<1, kFoldThreads, 0, ctx().stream>>>(parts, np,
-                                                                                                 (double *)res);
-                                   },
-                                   "msl_fold_sum_f32");
+    return run_fold<CompSumPolicy, true>(d_in, n, d_result, "msl_fold_sum_f32");
 }
 
 int msl_fold_sum_f32_host(const float *d_in, size_t n, double *h_sum64, float *h_sum32) {
-    int rc = run_fold<CompSumPolicy>(d_in, n, nullptr,
-                                     [](Pair *parts, int np, void *res) {
-                                         fold_sum_f32_stage2<<<1, kFoldThreads, 0, ctx().stream>>>(parts, np,
-                                                                                                   (double *)res);
-                                     },
-                                     "msl_fold_sum_f32_host");
+    int rc = run_fold<CompSumPolicy, true>(d_in, n, nullptr, "msl_fold_sum_f32_host");
     if (rc) return rc;
     void *ws = nullptr;
     rc = scratch_result(&ws);
@@ -259,12 +237,7 @@ int msl_fold_sum_f32_host(const float *d_in, size_t n, double *h_sum64, float *h
 
 int msl_fold_sum_u32_host(const uint32_t *d_in, size_t n, uint32_t *h_sum) {
     MSL_REQUIRE(h_sum != nullptr, "msl_fold_sum_u32_host: null result pointer");
-    int rc = run_fold<SumU32Policy>(d_in, n, nullptr,
-                                    [](uint32_t *parts, int np, void *res) {
-                                        fold_generic_stage2<SumU32Policy><<<1, kFoldThreads, 0, ctx().stream>>>(
-                                            parts, np, (uint32_t *)res);
-                                    },
-                                    "msl_fold_sum_u32_host");
+    int rc = run_fold<SumU32Policy, false>(d_in, n, nullptr, "msl_fold_sum_u32_host");
     if (rc) return rc;
     void *ws = nullptr;
     rc = scratch_result(&ws);
@@ -274,12 +247,7 @@ int msl_fold_sum_u32_host(const uint32_t *d_in, size_t n, uint32_t *h_sum) {
 
 int msl_fold_sum_i32_host(const int32_t *d_in, size_t n, int64_t *h_sum) {
     MSL_REQUIRE(h_sum != nullptr, "msl_fold_sum_i32_host: null result pointer");
-    int rc = run_fold<SumI32Policy>(d_in, n, nullptr,
-                                    [](long long *parts, int np, void *res) {
-                                        fold_generic_stage2<SumI32Policy><<<1, kFoldThreads, 0, ctx().stream>>>(
-                                            parts, np, (long long *)res);
-                                    },
-                                    "msl_fold_sum_i32_host");
+    int rc = run_fold<SumI32Policy, false>(d_in, n, nullptr, "msl_fold_sum_i32_host");
     if (rc) return rc;
     void *ws = nullptr;
     rc = scratch_result(&ws);
@@ -294,12 +262,7 @@ int msl_fold_sum_i32_host(const int32_t *d_in, size_t n, int64_t *h_sum) {
 int msl_fold_max_f32_host(const float *d_in, size_t n, float *h_max) {
     MSL_REQUIRE(h_max != nullptr, "msl_fold_max_f32_host: null result pointer");
     MSL_REQUIRE(n > 0, "msl_fold_max_f32_host: empty input");
-    int rc = run_fold<MaxF32Policy>(d_in, n, nullptr,
-                                    [](float *parts, int np, void *res) {
-                                        fold_generic_stage2<MaxF32Policy><<<1, kFoldThreads, 0, ctx().stream>>>(
-                                            parts, np, (float *)res);
-                                    },
-                                    "msl_fold_max_f32_host");
+    int rc = run_fold<MaxF32Policy, false>(d_in, n, nullptr, "msl_fold_max_f32_host");
     if (rc) return rc;
     void *ws = nullptr;
     rc = scratch_result(&ws);

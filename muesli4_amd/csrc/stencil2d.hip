@@ -48,20 +48,34 @@ struct NativeVec<int32_t> {
     typedef v4i type;
 };
 
-// Load 4 consecutive columns [c, c+4) of row pointer p (row already known to
-// be inside the grid); columns outside [0, cols) read as nv.
-template <typename T>
-__device__ __forceinline__ Vec4<T> load_quad(const T *__restrict__ p, int c, int cols, T nv, bool vec) {
+// Branch-free read of the 4 columns [c, c+4) of row yy: the row and the
+// columns are clamped into the grid so the load is always issued at a valid
+// address, and values outside the grid are replaced by nv afterwards.  With
+// VEC (cols % 4 == 0, 16-byte aligned rows, c % 4 == 0) a quad is either
+// wholly inside or wholly outside the grid and is read with one 16-byte load.
+template <typename T, bool VEC>
+__device__ __forceinline__ Vec4<T> read_quad(const T *__restrict__ in, int yy, int c, int rows, int cols, T nv) {
+    typedef typename NativeVec<T>::type V;
+    const int yc = min(max(yy, 0), rows - 1);
+    const bool rowok = yy == yc;
+    const T *rowp = in + (size_t)yc * cols;
     Vec4<T> q;
-    if (vec && c >= 0 && c + 3 < cols) {
-        typename NativeVec<T>::type v = *reinterpret_cast<const typename NativeVec<T>::type *>(p + c);
-        q.v[0] = v.x;
-        q.v[1] = v.y;
-        q.v[2] = v.z;
-        q.v[3] = v.w;
+    if constexpr (VEC) {
+        const int cc = min(max(c, 0), cols - 4);
+        const bool ok = rowok && cc == c;
+        const V v = *reinterpret_cast<const V *>(rowp + cc);
+        q.v[0] = ok ? v.x : nv;
+        q.v[1] = ok ? v.y : nv;
+        q.v[2] = ok ? v.z : nv;
+        q.v[3] = ok ? v.w : nv;
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; k++) q.v[k] = (c + k >= 0 && c + k < cols) ? p[c + k] : nv;
+        for (int k = 0; k < 4; k++) {
+            const int ck = c + k;
+            const int cc = min(max(ck, 0), cols - 1);
+            const T v = rowp[cc];
+            q.v[k] = (rowok && cc == ck) ? v : nv;
+        }
     }
     return q;
 }
@@ -112,155 +126,102 @@ struct LifePolicy {
 };
 
 template <int R>
-struct PrefetchDepth {
-    static constexpr int value = R <= 2 ? 8 : (R <= 5 ? 4 : 2);
-};
-
-template <int R>
 struct StripConsts {
     static constexpr int RP = (R + 3) / 4 * 4;     // halo padded to whole quads
     static constexpr int LDSW = kStripW + 2 * RP;  // elements per staged row
     static constexpr int NQ = (2 * RP) / 4 + 1;    // 16-byte window reads per lane
-    static constexpr int PER = 2 * R + 1;          // ring period
-    static constexpr int D = PrefetchDepth<R>::value;
-    static constexpr int U = PER * D;              // unroll: multiple of PER, D and 2
+    static constexpr int PER = 2 * R + 1;          // ring period (= loop unroll)
+    // Rows requested ahead of use: a divisor of PER (so queue slots stay
+    // static), larger where the per-row work is too short to hide latency.
+    static constexpr int D = PER <= 7 ? PER : (R > 10 ? 1 : (PER % 5 == 0 ? 5 : (PER % 3 == 0 ? 3 : 1)));
 };
 
-// March one wave down its (strip, segment).  INTERIOR: the strip plus halo
-// columns, the segment plus halo and prefetch rows, all lie inside the grid
-// and rows are 16-byte aligned, so loads and stores need no bounds checks;
-// otherwise every access is checked and reads outside the grid give nv.
-template <typename Pol, bool INTERIOR>
+// March one wave down its (strip, segment).  One code path for every tile:
+// reads are clamped (read_quad), stores are masked per lane, and the loop is
+// unrolled over exactly one ring period so the code stays small (the
+// instruction cache is shared by two CUs) while every register index is a
+// compile-time constant.  Row t+D is requested when row t is staged.
+template <typename Pol, bool VEC>
 __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
                                             int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
-                                            bool vecb, const Pol &pol,
-                                            typename Pol::T (*lds)[StripConsts<Pol::R>::LDSW]) {
+                                            const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R>::LDSW]) {
     typedef typename Pol::T T;
     typedef typename NativeVec<T>::type V;
     constexpr int R = Pol::R;
     constexpr int RP = StripConsts<R>::RP;
     constexpr int NQ = StripConsts<R>::NQ;
     constexpr int PER = StripConsts<R>::PER;
-    constexpr int D = StripConsts<R>::D;
-    constexpr int U = StripConsts<R>::U;
 
     const int lane = threadIdx.x & 63;
     const int x0 = xs + 4 * lane;
     const int total = (yend - y0) + 2 * R;  // input rows y0-R .. yend-1+R
     // Halo quads: left quads for lanes [0, RP/4), right quads for lanes
-    // [RP/4, RP/2).  In the interior path every other lane re-loads and
-    // re-stages its own centre quad instead, so no lane diverges.
+    // [RP/4, RP/2); every other lane re-reads and re-stages its own centre
+    // quad, so no lane diverges.
     const bool has_halo = lane < RP / 2;
-    const int hx = lane < RP / 4 ? xs - RP + 4 * lane : xs + kStripW + 4 * (lane - RP / 4);
-    const int hslot = lane < RP / 4 ? 4 * lane : RP + kStripW + 4 * (lane - RP / 4);
-    const int hx_i = has_halo ? hx : x0;
-    const int hslot_i = has_halo ? hslot : RP + 4 * lane;
+    const int hx = has_halo ? (lane < RP / 4 ? xs - RP + 4 * lane : xs + kStripW + 4 * (lane - RP / 4)) : x0;
+    const int hslot = has_halo ? (lane < RP / 4 ? 4 * lane : RP + kStripW + 4 * (lane - RP / 4)) : RP + 4 * lane;
+    const bool lane_in = x0 < cols;
 
-    Vec4<T> pc[D], ph[D];  // prefetched centre / halo quads
-    Vec4<T> ring[PER];     // row sums
+    constexpr int D = StripConsts<R>::D;
+    Vec4<T> ring[PER];  // row sums
     Vec4<T> cring[Pol::kCenter ? PER : 1];
-
-    auto fetch = [&](int t, Vec4<T> &c, Vec4<T> &h) {
-        const int yy = y0 - R + t;
-        if (INTERIOR) {
-            const T *rowp = in + (size_t)yy * cols;
-            V a = *reinterpret_cast<const V *>(rowp + x0);
-            V b = *reinterpret_cast<const V *>(rowp + hx_i);
-            c.v[0] = a.x; c.v[1] = a.y; c.v[2] = a.z; c.v[3] = a.w;
-            h.v[0] = b.x; h.v[1] = b.y; h.v[2] = b.z; h.v[3] = b.w;
-        } else if (t < total && yy >= 0 && yy < rows) {
-            const T *rowp = in + (size_t)yy * cols;
-            c = load_quad<T>(rowp, x0, cols, nv, vecb);
-            if (has_halo) h = load_quad<T>(rowp, hx, cols, nv, vecb);
-        } else {
+    Vec4<T> qc[D], qh[D];  // requested rows t .. t+D-1 (centre and halo quads)
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                c.v[k] = nv;
-                h.v[k] = nv;
+    for (int d = 0; d < D; d++) {
+        qc[d] = read_quad<T, VEC>(in, y0 - R + d, x0, rows, cols, nv);
+        qh[d] = read_quad<T, VEC>(in, y0 - R + d, hx, rows, cols, nv);
+    }
+
+    for (int tb = 0; tb < total; tb += PER) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int t = tb + u;  // ring slot of row t is u (tb is a multiple of PER)
+            const int qs = u % D;  // queue slot of row t
+            T *buf = lds[t & 1];
+            *reinterpret_cast<V *>(buf + RP + 4 * lane) = V{qc[qs].v[0], qc[qs].v[1], qc[qs].v[2], qc[qs].v[3]};
+            *reinterpret_cast<V *>(buf + hslot) = V{qh[qs].v[0], qh[qs].v[1], qh[qs].v[2], qh[qs].v[3]};
+            // request row t+D into the slot just staged
+            qc[qs] = read_quad<T, VEC>(in, y0 - R + t + D, x0, rows, cols, nv);
+            qh[qs] = read_quad<T, VEC>(in, y0 - R + t + D, hx, rows, cols, nv);
+            wave_lds_sync();
+            T w[4 * NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const V v = *reinterpret_cast<const V *>(buf + 4 * lane + 4 * q);
+                w[4 * q + 0] = v.x;
+                w[4 * q + 1] = v.y;
+                w[4 * q + 2] = v.z;
+                w[4 * q + 3] = v.w;
             }
-        }
-    };
-
-    // One row step.  k_slot = t % D (prefetch slot), k_ring = t % PER,
-    // k_buf = t & 1; after unrolling these are compile-time constants, so
-    // every register array index is static.  'store' emits output row t-2R.
-    auto step = [&](int t, int k_slot, int k_ring, int k_buf, bool store) {
-        T *buf = lds[k_buf];
-        // stage row t
-        *reinterpret_cast<V *>(buf + RP + 4 * lane) =
-            V{pc[k_slot].v[0], pc[k_slot].v[1], pc[k_slot].v[2], pc[k_slot].v[3]};
-        if (INTERIOR) {
-            *reinterpret_cast<V *>(buf + hslot_i) =
-                V{ph[k_slot].v[0], ph[k_slot].v[1], ph[k_slot].v[2], ph[k_slot].v[3]};
-        } else if (has_halo) {
-            *reinterpret_cast<V *>(buf + hslot) =
-                V{ph[k_slot].v[0], ph[k_slot].v[1], ph[k_slot].v[2], ph[k_slot].v[3]};
-        }
-        // prefetch row t + D into the slot just consumed
-        fetch(t + D, pc[k_slot], ph[k_slot]);
-        wave_lds_sync();
-        T w[4 * NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            V v = *reinterpret_cast<const V *>(buf + 4 * lane + 4 * q);
-            w[4 * q + 0] = v.x;
-            w[4 * q + 1] = v.y;
-            w[4 * q + 2] = v.z;
-            w[4 * q + 3] = v.w;
-        }
-        Vec4<T> rs;
+            for (int c = 0; c < 4; c++) ring[u].v[c] = pol.template rowsum<RP>(w, c);
+            if constexpr (Pol::kCenter) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) rs.v[c] = pol.template rowsum<RP>(w, c);
-        ring[k_ring] = rs;
-        if constexpr (Pol::kCenter) {
-#pragma unroll
-            for (int c = 0; c < 4; c++) cring[k_ring].v[c] = w[RP + c];
-        }
-        if (store) {
-            const int y = y0 - 2 * R + t;
-            Vec4<T> o;
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                T s = pol.colsum_init();
-#pragma unroll
-                for (int k = 0; k < PER; k++) s = pol.acc(s, ring[(k_ring + 1 + k) % PER].v[c]);
-                T centre = T(0);
-                if constexpr (Pol::kCenter) centre = cring[(k_ring + 1 + R) % PER].v[c];
-                o.v[c] = pol.finish(s, centre);
+                for (int c = 0; c < 4; c++) cring[u].v[c] = w[RP + c];
             }
-            T *orow = out + (size_t)y * cols;
-            if (INTERIOR || (vecb && x0 + 3 < cols)) {
-                __builtin_nontemporal_store(V{o.v[0], o.v[1], o.v[2], o.v[3]}, reinterpret_cast<V *>(orow + x0));
-            } else {
+            const int y = y0 - 2 * R + t;  // output row completed by row t
+            if (t >= 2 * R && y < yend) {
+                Vec4<T> o;
 #pragma unroll
-                for (int c = 0; c < 4; c++)
-                    if (x0 + c < cols) orow[x0 + c] = o.v[c];
-            }
-        }
-    };
-
+                for (int c = 0; c < 4; c++) {
+                    T sum = pol.colsum_init();
 #pragma unroll
-    for (int d = 0; d < D; d++) fetch(d, pc[d], ph[d]);
-
-    if constexpr (INTERIOR) {
-        // Branch-free schedule (the segment length is a multiple of U): 2R
-        // prologue rows without output, then whole unrolled blocks of U rows.
-        // Keeping conditionals out of the unrolled body lets the compiler's
-        // memory-counter waits track each prefetched row exactly instead of
-        // draining the prefetch queue at every step.
+                    for (int k = 0; k < PER; k++) sum = pol.acc(sum, ring[(u + 1 + k) % PER].v[c]);
+                    T centre = T(0);
+                    if constexpr (Pol::kCenter) centre = cring[(u + 1 + R) % PER].v[c];
+                    o.v[c] = pol.finish(sum, centre);
+                }
+                T *orow = out + (size_t)y * cols;
+                if constexpr (VEC) {
+                    if (lane_in)
+                        __builtin_nontemporal_store(V{o.v[0], o.v[1], o.v[2], o.v[3]},
+                                                    reinterpret_cast<V *>(orow + x0));
+                } else {
 #pragma unroll
-        for (int t = 0; t < 2 * R; t++) step(t, t % D, t % PER, t & 1, false);
-        for (int tb = 2 * R; tb < total; tb += U) {
-#pragma unroll
-            for (int u = 0; u < U; u++)
-                step(tb + u, (2 * R + u) % D, (2 * R + u) % PER, (2 * R + u) & 1, true);
-        }
-    } else {
-        for (int t0 = 0; t0 < total; t0 += U) {
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int t = t0 + u;
-                if (t < total) step(t, u % D, u % PER, u & 1, t >= 2 * R);
+                    for (int c = 0; c < 4; c++)
+                        if (x0 + c < cols) orow[x0 + c] = o.v[c];
+                }
             }
         }
     }
@@ -269,15 +230,13 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
 // One wave per (strip, segment) item, four independent waves per 256-thread
 // workgroup; each wave stages rows in its own LDS region, so only
 // wave-level ordering is needed (no workgroup barrier).
-template <typename Pol>
+template <typename Pol, bool VEC>
 __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__restrict__ out,
                                                             const typename Pol::T *__restrict__ in, int rows, int cols,
                                                             typename Pol::T nv, int seg, int strips_x, int nitems,
-                                                            int vec, Pol pol) {
+                                                            Pol pol) {
     typedef typename Pol::T T;
     constexpr int R = Pol::R;
-    constexpr int RP = StripConsts<R>::RP;
-    constexpr int D = StripConsts<R>::D;
     __shared__ __attribute__((aligned(16))) T lds[4][2][StripConsts<R>::LDSW];
 
     const int wave = threadIdx.x >> 6;
@@ -285,25 +244,16 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     if (item >= nitems) return;
     const int sx = item % strips_x;
     const int sy = item / strips_x;
-    const int xs = sx * kStripW;
     const int y0 = sy * seg;
-    const int yend = min(y0 + seg, rows);
-    const bool interior = vec != 0 && xs - RP >= 0 && xs + kStripW + RP <= cols && y0 - R >= 0 &&
-                          (long)yend + R + D <= (long)rows && (yend - y0) % StripConsts<R>::U == 0;
-    if (interior)
-        strip_march<Pol, true>(out, in, rows, cols, nv, xs, y0, yend, true, pol, lds[wave]);
-    else
-        strip_march<Pol, false>(out, in, rows, cols, nv, xs, y0, yend, vec != 0, pol, lds[wave]);
+    strip_march<Pol, VEC>(out, in, rows, cols, nv, sx * kStripW, y0, min(y0 + seg, rows), pol, lds[wave]);
 }
 
 struct StripShape {
     int seg, strips_x, segs, nitems;
 };
 
-// Segment length: about 16 waves per CU, rounded to a whole number of
-// unrolled blocks (unroll) so that interior segments take the branch-free
-// schedule.
-inline StripShape strip_shape(int rows, int cols, int seg, int unroll) {
+// Segment length: about 16 waves per CU.
+inline StripShape strip_shape(int rows, int cols, int seg) {
     StripShape s;
     s.strips_x = (cols + kStripW - 1) / kStripW;
     if (seg <= 0) seg = tune_int("MSL_STRIP_SEG", 0);
@@ -311,9 +261,7 @@ inline StripShape strip_shape(int rows, int cols, int seg, int unroll) {
         long target = (long)ctx().num_cus * 16;
         long segs = (target + s.strips_x - 1) / s.strips_x;
         seg = (int)((rows + segs - 1) / segs);
-        if (seg < 32) seg = 32;
-        seg = (seg + unroll / 2) / unroll * unroll;
-        if (seg < unroll) seg = unroll;
+        if (seg < 8) seg = 8;
     }
     s.seg = seg;
     s.segs = (rows + seg - 1) / seg;
@@ -324,10 +272,15 @@ inline StripShape strip_shape(int rows, int cols, int seg, int unroll) {
 template <typename Pol>
 int launch_strip(typename Pol::T *out, const typename Pol::T *in, int rows, int cols, typename Pol::T nv, Pol pol,
                  int seg, const char *name) {
-    StripShape s = strip_shape(rows, cols, seg, StripConsts<Pol::R>::U);
-    int vec = ((cols & 3) == 0) && ((((uintptr_t)in) & 15u) == 0) && ((((uintptr_t)out) & 15u) == 0);
-    strip_stencil_kernel<Pol><<<(s.nitems + 3) / 4, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg, s.strips_x,
-                                                                   s.nitems, vec, pol);
+    StripShape s = strip_shape(rows, cols, seg);
+    const bool vec = ((cols & 3) == 0) && ((((uintptr_t)in) & 15u) == 0) && ((((uintptr_t)out) & 15u) == 0);
+    const int blocks = (s.nitems + 3) / 4;
+    if (vec)
+        strip_stencil_kernel<Pol, true><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg, s.strips_x,
+                                                                           s.nitems, pol);
+    else
+        strip_stencil_kernel<Pol, false><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg, s.strips_x,
+                                                                            s.nitems, pol);
     return check_launch(name);
 }
 

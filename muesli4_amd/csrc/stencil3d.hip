@@ -5,13 +5,12 @@
 // (include/detail/map_kernels.cuh:140-149, include/plcube.h:105-119,
 // functor examples/heatdiffusion.cpp:55-65).
 //
-// Here a 256-thread workgroup owns a 64 x (4*RPT) column of the cube and
-// marches a segment of z-planes (2.5-D blocking): each thread keeps the z-1,
-// z and z+1 values of its RPT cells in registers (plane z+1 prefetched one
-// step ahead), the current plane plus a one-cell x/y halo is staged in LDS
-// (double-buffered, one barrier per plane), and the cell is
+// Here one wavefront owns a 64 x 8 column of the cube and marches a segment
+// of z-planes (2.5-D blocking) with all neighbours in registers: y and z
+// neighbours in the lane's own plane ring, x neighbours from the adjacent
+// lanes by DPP lane shifts; each plane is requested one step before its use.  Each cell
 //   v = 0 + in(x-1) + in(x+1) + in(y-1) + in(y+1) + in(z-1) + in(z+1); v / 6
-// in exactly that order.  Each cell is read from HBM once and written once
+// is computed in exactly that order.  Each cell is read from HBM once and written once
 // (8 bytes per cell per step); the optional fused residual folds
 // max |v - in| into one device scalar with an integer atomic max (the bit
 // patterns of non-negative floats order like unsigned integers).
@@ -23,11 +22,9 @@
 
 namespace mslhip {
 
-constexpr int kTX = 64;  // x extent of a tile (one wave row)
-constexpr int kWY = 4;   // waves per workgroup (y)
-constexpr int kRPT = 4;  // y rows per thread
-constexpr int kTY = kWY * kRPT;
-constexpr int kP = 8;    // plane ring: z-1 .. z+6 (planes z+2 .. z+6 in flight)
+constexpr int kTX = 64;  // x extent of a wave tile (one cell per lane)
+constexpr int kTY = 8;   // y extent of a wave tile (cells per lane)
+constexpr int kP = 4;    // plane ring: z-1, z, z+1 and the requested z+2
 
 // Branch-free cell read: the address is clamped into the cube and the value
 // replaced by nv when the cell lies outside, so loads are never predicated
@@ -41,106 +38,96 @@ __device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, in
     return (x == xc && y == yc && z == zc) ? v : nv;
 }
 
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane
+// i-1 (resp. i+1); the lane whose source is outside the wave keeps 'edge'.
+__device__ __forceinline__ float from_left(float v, float edge) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v, float edge) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+// One wave per (64 x kTY column tile, z-segment); four independent waves per
+// workgroup, no LDS and no barrier.  Lane l owns column x = x0 + l and keeps,
+// for a ring of kP planes, the kTY cells of its column plus the two y-halo
+// cells (rows y0-1 and y0+kTY) and one x-halo cell per row (lane 0 reads
+// x0-1, lane 63 reads x0+64, every other lane re-reads its own cell).  x
+// neighbours come from the adjacent lanes by DPP, y neighbours from the
+// lane's own registers, z neighbours from the plane ring.
 template <bool kResidual>
 __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, const float *__restrict__ in, int nx,
                                                     int ny, int nz, float nv, int zseg, int xtiles, int ytiles,
                                                     int nitems, float *__restrict__ maxdiff) {
-    __shared__ float tile[2][kTY + 2][kTX + 2];
-    __shared__ float wmax[kWY];
-    const int tx = threadIdx.x & 63;
-    const int ty = threadIdx.x >> 6;  // wave index
-    const int item = (int)xcd_tile(blockIdx.x, gridDim.x);
-    const int xt = item % xtiles;
-    const int rest = item / xtiles;
-    const int yt = rest % ytiles;
-    const int zs = rest / ytiles;
-    const int x = xt * kTX + tx;
-    const int ybase = yt * kTY;
-    const int z0 = zs * zseg;
-    const int z1 = min(z0 + zseg, nz);
-
-    // Halo duty for plane loads: wave 0 loads row ybase-1, wave 1 row
-    // ybase+kTY, wave 2 lanes 0..kTY-1 the column x=-1 side, lanes
-    // 32..32+kTY-1 the x=+kTX side.  Other lanes load a harmless duplicate
-    // of their own cell and skip the LDS write.
-    int hx = x, hy = ybase + ty, hsx = 0, hsy = 0;
-    bool hal = false;
-    if (ty == 0) {
-        hal = true; hy = ybase - 1; hsx = tx + 1; hsy = 0;
-    } else if (ty == 1) {
-        hal = true; hy = ybase + kTY; hsx = tx + 1; hsy = kTY + 1;
-    } else if (ty == 2 && tx < kTY) {
-        hal = true; hx = xt * kTX - 1; hy = ybase + tx; hsx = 0; hsy = tx + 1;
-    } else if (ty == 2 && tx >= 32 && tx < 32 + kTY) {
-        hal = true; hx = xt * kTX + kTX; hy = ybase + (tx - 32); hsx = kTX + 1; hsy = tx - 32 + 1;
-    }
-
-    // Ring of planes per cell: slot(zz) = (zz - z0 + 1) & (kP-1).
-    float q[kRPT][kP];
-    float hq[kP];
-#pragma unroll
-    for (int k = 0; k < kP - 1; k++) {
-        const int zz = z0 - 1 + k;
-#pragma unroll
-        for (int r = 0; r < kRPT; r++) q[r][k] = ld_cell(in, x, ybase + ty + r * kWY, zz, nx, ny, nz, nv);
-        hq[k] = ld_cell(in, hx, hy, zz, nx, ny, nz, nv);
-    }
+    const int lane = threadIdx.x & 63;
+    const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     float dmax = 0.0f;
-    // Whole blocks of kP planes; planes past z1 are computed but not stored.
-    for (int zb = z0; zb < z1; zb += kP) {
+    if (item < nitems) {
+        const int xt = item % xtiles;
+        const int rest = item / xtiles;
+        const int yt = rest % ytiles;
+        const int zs = rest / ytiles;
+        const int x = xt * kTX + lane;
+        const int y0 = yt * kTY;
+        const int z0 = zs * zseg;
+        const int z1 = min(z0 + zseg, nz);
+        const int hxl = lane == 0 ? x - 1 : (lane == 63 ? x + 1 : x);  // x-halo column
+
+        // slot(zz) = (zz - z0 + 1) & (kP-1); col[j] is row y0-1+j
+        float col[kTY + 2][kP];
+        float hal[kTY][kP];
+        auto load_plane = [&](int zz, int slot) {
 #pragma unroll
-        for (int u = 0; u < kP; u++) {
-            const int z = zb + u;
-            const int sb = u & (kP - 1);        // z-1
-            const int sc = (u + 1) & (kP - 1);  // z
-            const int sa = (u + 2) & (kP - 1);  // z+1
-            float (*tl)[kTX + 2] = tile[u & 1];
-            // stage plane z
+            for (int j = 0; j < kTY + 2; j++) col[j][slot] = ld_cell(in, x, y0 - 1 + j, zz, nx, ny, nz, nv);
 #pragma unroll
-            for (int r = 0; r < kRPT; r++) tl[ty + r * kWY + 1][tx + 1] = q[r][sc];
-            if (hal) tl[hsy][hsx] = hq[sc];
-            // prefetch plane z+6 into the slot of plane z+6 (= slot of z-2)
-            {
-                const int sn = (u + 7) & (kP - 1);
-                const int zz = z + 6;
+            for (int j = 0; j < kTY; j++) hal[j][slot] = ld_cell(in, hxl, y0 + j, zz, nx, ny, nz, nv);
+        };
 #pragma unroll
-                for (int r = 0; r < kRPT; r++) q[r][sn] = ld_cell(in, x, ybase + ty + r * kWY, zz, nx, ny, nz, nv);
-                hq[sn] = ld_cell(in, hx, hy, zz, nx, ny, nz, nv);
-            }
-            lds_barrier();
+        for (int k = 0; k < kP - 1; k++) load_plane(z0 - 1 + k, k);
+
+        // Whole blocks of kP planes; planes past z1 are computed, not stored.
+        for (int zb = z0; zb < z1; zb += kP) {
 #pragma unroll
-            for (int r = 0; r < kRPT; r++) {
-                const int ly = ty + r * kWY + 1;
-                const int y = ybase + ty + r * kWY;
-                float v = 0.0f;
-                v = __fadd_rn(v, tl[ly][tx]);          // x-1
-                v = __fadd_rn(v, tl[ly][tx + 2]);      // x+1
-                v = __fadd_rn(v, tl[ly - 1][tx + 1]);  // y-1
-                v = __fadd_rn(v, tl[ly + 1][tx + 1]);  // y+1
-                v = __fadd_rn(v, q[r][sb]);            // z-1
-                v = __fadd_rn(v, q[r][sa]);            // z+1
-                v = __fdiv_rn(v, 6.0f);
-                if (z < z1 && x < nx && y < ny) {
-                    __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
-                    if (kResidual) {
-                        float d = fabsf(__fsub_rn(v, q[r][sc]));
-                        dmax = d > dmax ? d : dmax;
+            for (int u = 0; u < kP; u++) {
+                const int z = zb + u;
+                const int sb = u & (kP - 1);        // z-1
+                const int sc = (u + 1) & (kP - 1);  // z
+                const int sa = (u + 2) & (kP - 1);  // z+1
+                // request plane z+2 into the slot of z-2 (free since step z-1)
+                load_plane(z + 2, (u + 3) & (kP - 1));
+#pragma unroll
+                for (int j = 0; j < kTY; j++) {
+                    const float c = col[j + 1][sc];
+                    float v = 0.0f;
+                    v = __fadd_rn(v, from_left(c, hal[j][sc]));   // x-1
+                    v = __fadd_rn(v, from_right(c, hal[j][sc]));  // x+1
+                    v = __fadd_rn(v, col[j][sc]);                 // y-1
+                    v = __fadd_rn(v, col[j + 2][sc]);             // y+1
+                    v = __fadd_rn(v, col[j + 1][sb]);             // z-1
+                    v = __fadd_rn(v, col[j + 1][sa]);             // z+1
+                    v = __fdiv_rn(v, 6.0f);
+                    const int y = y0 + j;
+                    if (z < z1 && x < nx && y < ny) {
+                        __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
+                        if (kResidual) {
+                            const float d = fabsf(__fsub_rn(v, c));
+                            dmax = d > dmax ? d : dmax;
+                        }
                     }
                 }
             }
         }
     }
     if (kResidual) {
+        __shared__ float wmax[4];
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) {
-            float o = __shfl_xor(dmax, m);
+            const float o = __shfl_xor(dmax, m);
             dmax = o > dmax ? o : dmax;
         }
-        if (tx == 0) wmax[ty] = dmax;
+        if (lane == 0) wmax[threadIdx.x >> 6] = dmax;
         __syncthreads();
         if (threadIdx.x == 0) {
-            float m = wmax[0];
-            for (int w = 1; w < kWY; w++) m = wmax[w] > m ? wmax[w] : m;
+            const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
             atomicMax(reinterpret_cast<unsigned int *>(maxdiff), __float_as_uint(m));
         }
     }
@@ -154,8 +141,8 @@ inline HeatShape heat_shape(int nx, int ny, int nz) {
     HeatShape s;
     s.xtiles = (nx + kTX - 1) / kTX;
     s.ytiles = (ny + kTY - 1) / kTY;
-    long cols = (long)s.xtiles * s.ytiles;
-    long target = (long)ctx().num_cus * 8;  // workgroups wanted
+    const long cols = (long)s.xtiles * s.ytiles;
+    const long target = (long)ctx().num_cus * 32;  // waves wanted
     long zsegs = (target + cols - 1) / cols;
     if (zsegs < 1) zsegs = 1;
     int zseg = (int)((nz + zsegs - 1) / zsegs);
@@ -177,10 +164,10 @@ int heat_step(float *out, const float *in, int nx, int ny, int nz, float nv, flo
     MSL_REQUIRE(out && in && out != in, "msl_heat7_f32: null or aliased buffers");
     HeatShape s = heat_shape(nx, ny, nz);
     if (maxdiff)
-        heat7_kernel<true><<<s.nitems, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
+        heat7_kernel<true><<<(s.nitems + 3) / 4, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
                                                                 s.nitems, maxdiff);
     else
-        heat7_kernel<false><<<s.nitems, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
+        heat7_kernel<false><<<(s.nitems + 3) / 4, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
                                                                  s.nitems, nullptr);
     return check_launch("msl_heat7_f32");
 }
