@@ -43,7 +43,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
+    ap.add_argument("--kernel", default=None,
+                    help="run one kernel only, --reps times, for profiler counter runs: "
+                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | heat7 | gol")
+    ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
+    if args.kernel:
+        run_single_kernel(args.kernel, args.reps)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part "
               "of this build", file=sys.stderr)
@@ -110,42 +117,46 @@ def main():
         step()
     hip.synchronize()
     step_ms = []
-    blur_ms = []
+    blur_ms = {r: [] for r in RADII}
     for _ in range(args.steps):
         fresh()
         hip.synchronize()
         t0 = time.perf_counter()
-        call("msl_timer_start")
         call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0)
         call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n)
         call("msl_mapindex_lin_f32", c.ptr, ROWS, COLS)
         call("msl_fold_sum_f32", a.ptr, n, d_res.ptr)
-        call("msl_timer_stop", C.byref(ms))
-        elem_ms = ms.value
-        call("msl_timer_start")
         for r in RADII:
+            # events on the library stream around the 20 graph-replayed steps
+            call("msl_timer_start")
             call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, r, 0.0, BLUR_ITERS, C.byref(inb))
-        call("msl_timer_stop", C.byref(ms))
+            call("msl_timer_stop", C.byref(ms))
+            blur_ms[r].append(ms.value)
         hip.synchronize()
         t1 = time.perf_counter()
-        blur_ms.append(ms.value)
         step_ms.append((t1 - t0) * 1e3)
+    med = {r: float(np.median(v)) for r, v in blur_ms.items()}
     blur_bytes = 8 * n * BLUR_ITERS * len(RADII)
-    blur_t = float(np.median(blur_ms)) * 1e-3
-    value = blur_bytes / blur_t / 1e9
-    log(f"[bench] headline blur GB/s {value:.1f} step ms {np.median(step_ms):.3f}")
+    value = blur_bytes / (sum(med.values()) * 1e-3) / 1e9
+    per_radius = {f"r{r}": {"ms_per_iteration": round(t / BLUR_ITERS, 5),
+                            "GBs": round(8 * n * BLUR_ITERS / (t * 1e-3) / 1e9, 1)} for r, t in med.items()}
+    log(f"[bench] headline blur GB/s {value:.1f} per radius {per_radius} step ms {np.median(step_ms):.3f}")
 
-    # ---- roofline of the dominant kernel (mean blur r=1) ---------------------
-    dom = "blur_r1"
-    achieved = kernel_bytes[dom] / (kernel_ms[dom] * 1e-3) / 1e9
+    # ---- roofline of the dominant kernel: the radius with the largest share
+    # of the timed region; average launch duration = its 20-step time / 20.
+    dom_r = max(med, key=lambda r: med[r])
+    launch_ms = med[dom_r] / BLUR_ITERS
+    achieved = 8 * n / (launch_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_blur_r1_8192.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_blur_r{dom_r}_8192.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": "strip_stencil_kernel<MeanPolicy<1>> (8192x8192, 8 B/cell)",
+    roofline = {"bound": "hbm",
+                "kernel": f"strip_stencil_kernel<MeanPolicy<{dom_r}>,true> (8192x8192 ping-pong, 8 B/cell)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": 8 * n, "avg_launch_ms": round(launch_ms, 5)}
 
     result = {
         "metric": "stencil GB/s (% HBM peak)",
@@ -165,6 +176,7 @@ def main():
                    "rows": ROWS, "cols": COLS, "radii": list(RADII), "iterations": BLUR_ITERS},
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 1),
         "roofline": roofline,
+        "per_radius": per_radius,
         "kernels": kernels,
         "device": info,
     }
@@ -178,6 +190,58 @@ def main():
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
+
+
+def run_single_kernel(name, reps):
+    """Launch one kernel of the headline workloads `reps` times (after two
+    warm-up launches) so that profiler runs see exactly that kernel.  Stencil
+    kernels ping-pong between two grids as in the iteration loops."""
+    from muesli4_amd import hip
+    from muesli4_amd.hip import DeviceArray, call
+
+    hip.init(0)
+    if name == "heat7":
+        nx = ny = nz = 512
+        n = nx * ny * nz
+        a = DeviceArray(n, np.float32)
+        b = DeviceArray(n, np.float32)
+        call("msl_fill_uniform_f32", a.ptr, n, SEED, float(np.float32(1.0 / 16777216.0)))
+        bufs = [a, b]
+        for i in range(reps + 2):
+            call("msl_heat7_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, nx, ny, nz, 0.0)
+    elif name == "gol":
+        rows = cols = 1024
+        n = rows * cols
+        a = DeviceArray(n, np.int32)
+        b = DeviceArray(n, np.int32)
+        call("msl_fill_bernoulli_i32", a.ptr, n, SEED, int(0.3 * 2.0 ** 64))
+        bufs = [a, b]
+        for i in range(reps + 2):
+            call("msl_gol_i32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, rows, cols)
+    else:
+        n = ROWS * COLS
+        a = DeviceArray(n, np.float32)
+        b = DeviceArray(n, np.float32)
+        c = DeviceArray(n, np.float32)
+        d_res = DeviceArray(1, np.float64)
+        scale = float(np.float32(255.0 / 16777216.0))
+        call("msl_fill_uniform_f32", a.ptr, n, SEED, scale)
+        call("msl_fill_uniform_f32", b.ptr, n, SEED + 1, scale)
+        bufs = [a, b]
+        for i in range(reps + 2):
+            if name == "map":
+                call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0)
+            elif name == "zip":
+                call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n)
+            elif name == "fold":
+                call("msl_fold_sum_f32", a.ptr, n, d_res.ptr)
+            elif name.startswith("blur_r"):
+                r = int(name[len("blur_r"):])
+                call("msl_stencil2d_mean_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, ROWS, COLS, r, 0.0)
+            else:
+                raise SystemExit(f"unknown kernel {name}")
+    hip.synchronize()
+    log(f"[bench] ran {name} x{reps + 2}")
 
 
 def bench_gol(hip, call, DeviceArray):
