@@ -18,6 +18,10 @@ LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(ROOT, "_build", "obj")
 ORACLE = os.path.join(ROOT, "oracle")
 
+INCLUDE = os.path.join(ROOT, "include")
+API_INCLUDE = os.path.join(PKG, "include")
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+
 LIB_NAME = "libmsl_hip.so"
 SOURCES = [
     "runtime.hip",
@@ -33,7 +37,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # -ffp-contract=off: no FMA contraction (bit-exact with the CPU loops).
 # packed FP32 instructions are disabled for all device code.
-HIP_FLAGS = [
+CODEGEN_FLAGS = [
     "-O3",
     "-std=c++17",
     f"--offload-arch={ARCH}",
@@ -41,10 +45,8 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall",
-    "-Wno-unused-function",
-    "-Wno-unused-variable",
 ]
+HIP_FLAGS = [*CODEGEN_FLAGS, "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 
 
 def _run(cmd, cwd=None):
@@ -84,9 +86,40 @@ def build_oracle() -> str:
     return os.path.join(ORACLE, "liboracle.so")
 
 
+def program_flags() -> list:
+    """hipcc flags for a program written against the msl:: class templates
+    (muesli4_amd/include): the same code-generation flags as the library,
+    the two include directories and the link line for libmsl_hip.so."""
+    return [*CODEGEN_FLAGS, "-x", "hip", "-I", API_INCLUDE, "-I", INCLUDE,
+            "-L", LIBDIR, "-lmsl_hip", f"-Wl,-rpath,{LIBDIR}"]
+
+
+def build_program(src: str, out: str, rpath_origin: str | None = None) -> str:
+    """Compile one C++ skeleton program (e.g. an example written against the
+    msl:: API) into an executable linked to libmsl_hip.so.  rpath_origin, if
+    given, is the library directory relative to the executable ($ORIGIN), so
+    the executable keeps working when the tree is moved."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    flags = program_flags()
+    if rpath_origin is not None:
+        flags = [f for f in flags if not f.startswith("-Wl,-rpath")] + [f"-Wl,-rpath,$ORIGIN/{rpath_origin}"]
+    tmp = out + ".tmp"
+    _run([HIPCC, *flags, src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cpp_tests() -> str:
+    """The C++ API test driver run by tests/test_gpu_cpp_api.py."""
+    out = os.path.join(CPP_TESTS, "build", "skeletons_test")
+    rel = os.path.relpath(LIBDIR, os.path.dirname(out))
+    return build_program(os.path.join(CPP_TESTS, "skeletons_test.cpp"), out, rpath_origin=rel)
+
+
 def build_all() -> None:
     build_oracle()
     build_hip()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":
