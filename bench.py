@@ -30,6 +30,9 @@ ROWS = COLS = 8192
 RADII = (1, 5, 10)
 BLUR_ITERS = 20
 SEED = 20240611
+BATCH_GRIDS = 4096
+BATCH_ITERS = 10
+BATCH_SEED = 2024
 
 
 def log(msg):
@@ -45,7 +48,7 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
     ap.add_argument("--kernel", default=None,
                     help="run one kernel only, --reps times, for profiler counter runs: "
-                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | heat7 | gol")
+                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | heat7 | gol | batch")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     if args.kernel:
@@ -184,6 +187,7 @@ def main():
     if not args.only_headline:
         result["gol"] = bench_gol(hip, call, DeviceArray)
         result["heat3d"] = bench_heat(hip, call, DeviceArray)
+        result["batched"] = bench_batch(hip)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(value)
@@ -200,6 +204,17 @@ def run_single_kernel(name, reps):
     from muesli4_amd.hip import DeviceArray, call
 
     hip.init(0)
+    if name == "batch":
+        from muesli4_amd.batch import BatchArena
+        from muesli4_amd.workloads import batch_shapes
+        rows, cols, radii = batch_shapes(BATCH_GRIDS, BATCH_SEED)
+        bt = BatchArena(rows, cols, radii, nv=0.0, seed=BATCH_SEED)
+        bt.autotune(None)
+        for _ in range(max(1, reps // BATCH_ITERS)):
+            bt.run(BATCH_ITERS)
+        hip.synchronize()
+        log(f"[bench] ran batch x{max(1, reps // BATCH_ITERS)} runs of {BATCH_ITERS} steps")
+        return
     if name == "heat7":
         nx = ny = nz = 512
         n = nx * ny * nz
@@ -300,6 +315,55 @@ def bench_heat(hip, call, DeviceArray):
            "ms": round(t * 1e3, 3), "GBs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
            "residuals": [float(x) for x in res]}
     log(f"[bench] heat {out['GBs']} GB/s")
+    return out
+
+
+def bench_batch(hip):
+    """4096 independent grids (long side log-uniform in [32,1024], aspect up
+    to 4:1, r uniform in 1..8), 10 mean-blur steps each plus a per-grid
+    fold(sum): persistent launches in one graph vs the plain loop of one
+    launch per grid per step and one fold per grid."""
+    from muesli4_amd.batch import BatchArena
+    from muesli4_amd.workloads import batch_shapes
+
+    rows, cols, radii = batch_shapes(BATCH_GRIDS, BATCH_SEED)
+    bt = BatchArena(rows, cols, radii, nv=0.0, seed=BATCH_SEED)
+    t0 = time.perf_counter()
+    tuned = bt.autotune(None)
+    tune_s = time.perf_counter() - t0
+    ms = C.c_float()
+
+    def timed(fn, reps):
+        out = []
+        for _ in range(reps):
+            bt.reset()
+            call = hip.call
+            call("msl_timer_start")
+            fn()
+            call("msl_timer_stop", C.byref(ms))
+            out.append(ms.value)
+        return float(np.median(out))
+
+    bt.run(BATCH_ITERS)  # capture the graph
+    persistent_ms = timed(lambda: bt.run(BATCH_ITERS), 5)
+    bt.run_loop(BATCH_ITERS)  # warm the single-grid kernels
+    loop_ms = timed(lambda: bt.run_loop(BATCH_ITERS), 2)
+    nbytes = bt.algorithmic_bytes(BATCH_ITERS)
+    info = bt.info()
+    tiles = {f"r{r}": [bt.tile(r, wc) for wc in range(4)] for r in range(1, 9)}
+    out = {"workload": f"{BATCH_GRIDS} DM<float> grids, sides log-uniform [32,1024], aspect 1-4, r in 1..8, "
+                       f"{BATCH_ITERS} mean-blur steps + per-grid compensated fold(sum)",
+           "grids": BATCH_GRIDS, "elements": bt.elements(), "algorithmic_bytes": nbytes,
+           "persistent_ms": round(persistent_ms, 3), "loop_ms": round(loop_ms, 3),
+           "speedup_vs_loop": round(loop_ms / persistent_ms, 2),
+           "persistent_GBs": round(nbytes / (persistent_ms * 1e-3) / 1e9, 1),
+           "loop_GBs": round(nbytes / (loop_ms * 1e-3) / 1e9, 1),
+           "pct_hbm_peak": round(100 * nbytes / (persistent_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
+           "launches_loop": int(BATCH_GRIDS * (BATCH_ITERS + 1)), "launches_persistent_graph": BATCH_ITERS + 1,
+           "autotune_s": round(tune_s, 2), "autotuned_classes": tuned, "launch": info,
+           "tiles_cols_per_lane_seg_by_width_class": tiles}
+    log(f"[bench] batched {out}")
+    bt.close()
     return out
 
 

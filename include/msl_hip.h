@@ -160,6 +160,54 @@ int msl_graph_destroy(void *graph);
 int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches);
 
 /* ------------------------------------------------------------------ */
+/* Batched mean blur.  No single reference counterpart: it replaces a loop
+ * of DM::mapStencil (src/dm.cpp:1535-1591) over many grids, one launch and
+ * synchronisation per grid per step, followed by DS::fold
+ * (src/ds.cpp:739-912) per grid.
+ *
+ * A batch is a fixed list of independent grids, each with its own shape,
+ * radius (1..10) and neutral value, and two device buffers a and b (step 0
+ * reads a and writes b, step 1 reads b and writes a, ...).  Every output
+ * is bit-identical to msl_stencil2d_mean_f32 on that grid alone.
+ *
+ * msl_batch_run: each step is one persistent launch over a device work
+ * queue of (grid, strip, segment) items; the steps and the per-grid fold
+ * are captured once into a HIP graph (per iteration count and d_sums) and
+ * replayed.  d_sums (nullable; ngrids doubles) receives the compensated
+ * fold(sum) of each grid's final state (iters >= 1).
+ * msl_batch_run_loop: the plain loop, one single-grid launch per grid per
+ * step and one msl_fold_sum_f32 per grid (the comparison baseline).
+ * Buffers must not be freed while the batch exists.                      */
+typedef struct msl_batch_grid {
+    float *a;   /* initial state (device)                     */
+    float *b;   /* second buffer (device), same size          */
+    int rows;
+    int cols;
+    int r;      /* mean-blur radius, 1..10                    */
+    float nv;   /* neutral value read outside the grid        */
+} msl_batch_grid;
+
+int msl_batch_create(const msl_batch_grid *h_grids, int ngrids, void **h_batch);
+int msl_batch_destroy(void *batch);
+/* Tile shape of one (radius, width class) cell: width classes are
+ * cols < 96, < 192, < 384 and >= 384; cols_per_lane 4 (256-column wave
+ * strips) or 1 (64-column strips); seg = rows per work item.           */
+int msl_batch_set_tile(void *batch, int r, int width_class, int cols_per_lane, int seg);
+int msl_batch_get_tile(const void *batch, int r, int width_class, int *h_cols_per_lane, int *h_seg);
+/* Persistent launch shape: workgroups per CU (capped by occupancy) and
+ * static round-robin (0) or atomic work queue (1).                    */
+int msl_batch_set_launch(void *batch, int blocks_per_cu, int dynamic_queue);
+/* Autotuner: times every tile candidate per (radius, width class) present
+ * in the batch, then the launch shape, and keeps the fastest.  With
+ * cache_path, a table saved for this device is loaded instead (then
+ * *h_entries is minus the number of entries loaded), else the tuned table
+ * is saved there.                                                       */
+int msl_batch_autotune(void *batch, const char *cache_path, int *h_entries);
+int msl_batch_info(const void *batch, int *h_items, int *h_blocks, int *h_blocks_per_cu, int *h_dynamic);
+int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b);
+int msl_batch_run_loop(void *batch, int iters, double *d_sums, int *h_result_in_b);
+
+/* ------------------------------------------------------------------ */
 /* Event timing on the library stream (for benchmarks). */
 int msl_timer_start(void);
 int msl_timer_stop(float *h_ms);

@@ -1,0 +1,676 @@
+// Batched mean blur: thousands of independent grids of different shapes and
+// radii stencilled together, each step one persistent launch over a device
+// work queue, the iteration loop and the per-grid fold captured in one HIP
+// graph.
+//
+// Reference counterpart: none as one operation.  A Muesli program runs
+// DM::mapStencil (src/dm.cpp:1535-1591) and DS::fold (src/ds.cpp:739-912)
+// grid by grid, each call a launch plus synchronisations; the plain loop
+// kept here for comparison (msl_batch_run_loop) does one launch per grid per
+// step and one fold per grid.
+//
+// Work items.  A grid is cut into wave strips of 64*C columns (C = 4 or 1
+// columns per lane) and segments of `seg` rows; one item = one (grid, strip,
+// segment), processed by one wavefront with the same strip march as the
+// single-grid kernel (so every output is bit-identical to it).  Items are
+// ordered by (radius, row mode, grid) so that waves that run at the same
+// time execute the same specialisation (the instruction cache is shared by
+// two CUs).  The persistent launch holds `blocks_per_cu` 4-wave workgroups
+// per CU; wave w takes items w, w+W, ... (static) or, with the dynamic
+// queue, its first item statically and every further item from an atomic
+// counter.  Consecutive items map to waves of one XCD (xcd_tile), so the
+// halo rows shared by neighbouring segments are served by one L2.
+//
+// Fold.  On the last step every lane keeps a TwoSum-compensated sum of the
+// values it stores; a wave combines its lanes in a fixed butterfly and
+// writes one (s, c) pair per item; a final kernel adds a grid's pairs in
+// FP64 in item order.  Deterministic for a given tiling.
+//
+// Autotuner.  Tile shape (C, seg) is chosen per (radius, width class) by
+// timing one step over the sub-batch of that class for every candidate;
+// the launch shape (workgroups per CU, static or dynamic queue) is then
+// timed over the whole batch.  The table can be saved to and loaded from a
+// text file keyed by the device name.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "strip.hpp"
+
+namespace mslhip {
+
+int mean_step(float *out, const float *in, int rows, int cols, int r, float nv, int seg);
+
+constexpr int kBatchMaxR = 10;
+constexpr int kBatchMaxR4 = 8;  // largest radius run with 4 columns per lane
+constexpr int kWidthClasses = 4;
+constexpr int kKindAligned4 = 0;    // C = 4, aligned rows
+constexpr int kKindUnaligned4 = 1;  // C = 4, dword-aligned rows
+constexpr int kKindNarrow1 = 2;     // C = 1
+
+struct BGrid {
+    float *a;
+    float *b;
+    int rows, cols, r, kind;
+    float nv;
+    int item0, nitems, pad;
+};
+
+struct BItem {
+    int grid, xs, y0, y1;
+};
+
+// Prefetch depth per (R, C) in the batched kernel: divisors of 2R+1 chosen
+// so that the largest specialisation stays within 128 VGPRs (four
+// workgroups per CU); narrow strips keep the whole ring period in flight.
+template <int R, int C>
+struct BatchDepth {
+    static constexpr int PER = 2 * R + 1;
+    static constexpr int D = C == 1 ? PER : (PER <= 5 ? PER : (PER % 3 == 0 ? 3 : 1));
+};
+
+inline int width_class(int cols) {
+    if (cols < 96) return 0;
+    if (cols < 192) return 1;
+    if (cols < 384) return 2;
+    return 3;
+}
+
+template <int R, bool kFold>
+__device__ __forceinline__ void batch_item(const BGrid &g, const BItem &it, int parity, float *ldsw, LaneSum *ls) {
+    const float *src = parity ? g.b : g.a;
+    float *dst = parity ? g.a : g.b;
+    const MeanPolicy<R> pol = MeanPolicy<R>::make();
+    if constexpr (R > kBatchMaxR4) {
+        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
+            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
+            reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls);
+    } else if (g.kind == kKindAligned4) {
+        strip_march<MeanPolicy<R>, 4, kAligned, kFold, BatchDepth<R, 4>::D>(
+            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
+            reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls);
+    } else if (g.kind == kKindUnaligned4) {
+        strip_march<MeanPolicy<R>, 4, kUnaligned, kFold, BatchDepth<R, 4>::D>(
+            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
+            reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls);
+    } else {
+        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
+            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
+            reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls);
+    }
+}
+
+constexpr int kBatchLdsw = StripConsts<kBatchMaxR4, 4>::LDSW > StripConsts<kBatchMaxR, 1>::LDSW
+                                 ? StripConsts<kBatchMaxR4, 4>::LDSW
+                                 : StripConsts<kBatchMaxR, 1>::LDSW;
+
+template <bool kFold>
+__global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict__ grids,
+                                                         const BItem *__restrict__ items, int nitems, int parity,
+                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
+    __shared__ __attribute__((aligned(16))) float lds[4][2 * kBatchLdsw];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * 4;
+    int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave);
+    while (i < nitems) {
+        const BItem it = items[i];
+        const BGrid g = grids[it.grid];
+        LaneSum ls;
+        float *ldsw = lds[wave];
+        switch (g.r) {
+            case 1: batch_item<1, kFold>(g, it, parity, ldsw, &ls); break;
+            case 2: batch_item<2, kFold>(g, it, parity, ldsw, &ls); break;
+            case 3: batch_item<3, kFold>(g, it, parity, ldsw, &ls); break;
+            case 4: batch_item<4, kFold>(g, it, parity, ldsw, &ls); break;
+            case 5: batch_item<5, kFold>(g, it, parity, ldsw, &ls); break;
+            case 6: batch_item<6, kFold>(g, it, parity, ldsw, &ls); break;
+            case 7: batch_item<7, kFold>(g, it, parity, ldsw, &ls); break;
+            case 8: batch_item<8, kFold>(g, it, parity, ldsw, &ls); break;
+            case 9: batch_item<9, kFold>(g, it, parity, ldsw, &ls); break;
+            default: batch_item<10, kFold>(g, it, parity, ldsw, &ls); break;
+        }
+        if constexpr (kFold) {
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) {
+                const float os = __shfl_xor(ls.s, m), oc = __shfl_xor(ls.c, m);
+                float s, e;
+                two_sum(ls.s, os, s, e);
+                ls.c = __fadd_rn(__fadd_rn(ls.c, oc), e);
+                ls.s = s;
+            }
+            if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
+        }
+        if (ctr) {
+            unsigned nx = 0;
+            if (lane == 0) nx = atomicAdd(ctr, 1u);
+            i = nw + __builtin_amdgcn_readfirstlane((int)nx);
+        } else {
+            i += nw;
+        }
+    }
+}
+
+// One wave per grid: sum = sum over the grid's items, in item order, of
+// (double)s + (double)c; lanes take items l, l+64, ... and are combined in a
+// fixed butterfly.
+__global__ __launch_bounds__(256) void batch_fold_final_kernel(const BGrid *__restrict__ grids, int ngrids,
+                                                               const float2 *__restrict__ partials,
+                                                               double *__restrict__ sums) {
+    const int lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gi >= ngrids) return;
+    const BGrid g = grids[gi];
+    double acc = 0.0;
+    for (int k = lane; k < g.nitems; k += 64) {
+        const float2 p = partials[g.item0 + k];
+        acc += (double)p.x;
+        acc += (double)p.y;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
+    if (lane == 0) sums[gi] = acc;
+}
+
+__global__ void batch_zero_kernel(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void batch_zero_f64_kernel(double *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ---- host side --------------------------------------------------------------
+struct Tile {
+    int c = 4;     // columns per lane (4 or 1)
+    int seg = 64;  // rows per item
+};
+
+struct Batch {
+    std::vector<msl_batch_grid> grids;
+    Tile table[kBatchMaxR + 1][kWidthClasses];
+    int blocks_per_cu = 2;
+    int dynamic = 0;
+    int max_blocks_per_cu = 4;
+    // device state
+    BGrid *d_grids = nullptr;
+    BItem *d_items = nullptr;
+    float2 *d_partials = nullptr;
+    unsigned *d_ctr = nullptr;  // one counter per step
+    int ctr_cap = 0;
+    int nitems = 0, items_cap = 0;
+    std::map<std::pair<int, const void *>, hipGraphExec_t> graphs;
+};
+
+static void drop_graphs(Batch &b) {
+    if (b.graphs.empty()) return;
+    (void)hipStreamSynchronize(ctx().stream);
+    for (auto &kv : b.graphs) (void)hipGraphExecDestroy(kv.second);
+    b.graphs.clear();
+}
+
+static int grid_kind(const msl_batch_grid &g, int c) {
+    if (c == 1 || g.cols < 4 || g.r > kBatchMaxR4) return kKindNarrow1;
+    const uintptr_t a = (uintptr_t)g.a | (uintptr_t)g.b;
+    if ((g.cols & 3) == 0 && (a & 15u) == 0) return kKindAligned4;
+    return kKindUnaligned4;
+}
+
+// Items and device descriptors of the grids selected by `pick` (all when
+// empty), with the given tiles.  Descriptors of unselected grids get no
+// items.
+static void plan_items(const Batch &b, const std::vector<int> &pick, const Tile (*table)[kWidthClasses],
+                       std::vector<BGrid> &hg, std::vector<BItem> &hi) {
+    const int n = (int)b.grids.size();
+    hg.assign(n, BGrid{});
+    hi.clear();
+    std::vector<int> order;
+    if (pick.empty()) {
+        order.resize(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+    } else {
+        order = pick;
+    }
+    std::vector<int> kind(n);
+    for (int g : order) {
+        const msl_batch_grid &G = b.grids[g];
+        kind[g] = grid_kind(G, table[G.r][width_class(G.cols)].c);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (b.grids[x].r != b.grids[y].r) return b.grids[x].r < b.grids[y].r;
+        return kind[x] < kind[y];
+    });
+    for (int g = 0; g < n; g++) {
+        const msl_batch_grid &G = b.grids[g];
+        hg[g] = BGrid{G.a, G.b, G.rows, G.cols, G.r, 0, G.nv, 0, 0, 0};
+    }
+    for (int g : order) {
+        const msl_batch_grid &G = b.grids[g];
+        const Tile t = table[G.r][width_class(G.cols)];
+        const int k = kind[g];
+        const int sw = 64 * (k == kKindNarrow1 ? 1 : 4);
+        const int seg = std::max(1, t.seg);
+        BGrid &d = hg[g];
+        d.kind = k;
+        d.item0 = (int)hi.size();
+        if (G.rows > 0 && G.cols > 0) {
+            for (int xs = 0; xs < G.cols; xs += sw)
+                for (int y0 = 0; y0 < G.rows; y0 += seg) hi.push_back(BItem{g, xs, y0, std::min(y0 + seg, G.rows)});
+        }
+        d.nitems = (int)hi.size() - d.item0;
+    }
+}
+
+static int upload(Batch &b, const std::vector<BGrid> &hg, const std::vector<BItem> &hi) {
+    drop_graphs(b);
+    const int ni = (int)hi.size();
+    if (ni > b.items_cap) {
+        if (b.d_items) MSL_HIP_TRY(hipFree(b.d_items));
+        if (b.d_partials) MSL_HIP_TRY(hipFree(b.d_partials));
+        b.d_items = nullptr;
+        b.d_partials = nullptr;
+        const int cap = std::max(ni, 1);
+        MSL_HIP_TRY(hipMalloc(&b.d_items, sizeof(BItem) * cap));
+        MSL_HIP_TRY(hipMalloc(&b.d_partials, sizeof(float2) * cap));
+        b.items_cap = cap;
+    }
+    if (!hg.empty())
+        MSL_HIP_TRY(hipMemcpyAsync(b.d_grids, hg.data(), sizeof(BGrid) * hg.size(), hipMemcpyHostToDevice,
+                                   ctx().stream));
+    if (ni) MSL_HIP_TRY(hipMemcpyAsync(b.d_items, hi.data(), sizeof(BItem) * ni, hipMemcpyHostToDevice, ctx().stream));
+    MSL_HIP_TRY(hipStreamSynchronize(ctx().stream));
+    b.nitems = ni;
+    return MSL_OK;
+}
+
+static int rebuild(Batch &b) {
+    std::vector<BGrid> hg;
+    std::vector<BItem> hi;
+    plan_items(b, {}, b.table, hg, hi);
+    return upload(b, hg, hi);
+}
+
+static int launch_blocks(const Batch &b) {
+    const int want = (b.nitems + 3) / 4;
+    const int cap = ctx().num_cus * std::max(1, std::min(b.blocks_per_cu, b.max_blocks_per_cu));
+    return std::max(1, std::min(want, cap));
+}
+
+static int ensure_ctr(Batch &b, int n) {
+    if (n <= b.ctr_cap) return MSL_OK;
+    drop_graphs(b);
+    if (b.d_ctr) MSL_HIP_TRY(hipFree(b.d_ctr));
+    b.d_ctr = nullptr;
+    MSL_HIP_TRY(hipMalloc(&b.d_ctr, sizeof(unsigned) * n));
+    b.ctr_cap = n;
+    return MSL_OK;
+}
+
+// Enqueues `iters` persistent steps (the last with the fused fold when
+// d_sums is given) and the fold combine.
+static int enqueue_run(Batch &b, int iters, double *d_sums) {
+    const int ng = (int)b.grids.size();
+    const int blocks = launch_blocks(b);
+    hipStream_t s = ctx().stream;
+    if (b.dynamic) {
+        batch_zero_kernel<<<(iters + 255) / 256, 256, 0, s>>>(b.d_ctr, iters);
+        int rc = check_launch("msl_batch_run");
+        if (rc) return rc;
+    }
+    for (int it = 0; it < iters; it++) {
+        unsigned *ctr = b.dynamic ? b.d_ctr + it : nullptr;
+        const bool fold = d_sums && it == iters - 1;
+        if (fold)
+            batch_step_kernel<true><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr);
+        else
+            batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr);
+        int rc = check_launch("msl_batch_run");
+        if (rc) return rc;
+    }
+    if (d_sums) {
+        batch_fold_final_kernel<<<(ng + 3) / 4, 256, 0, s>>>(b.d_grids, ng, b.d_partials, d_sums);
+        return check_launch("msl_batch_run");
+    }
+    return MSL_OK;
+}
+
+// Mean time of `reps` back-to-back steps (parity 0, no fold), in ms.
+static int time_steps(Batch &b, int reps, float *ms) {
+    int rc = ensure_ctr(b, reps);
+    if (rc) return rc;
+    const int blocks = launch_blocks(b);
+    hipStream_t s = ctx().stream;
+    for (int w = 0; w < 2; w++) {  // warm-up: code, TLB, caches
+        if (b.dynamic) batch_zero_kernel<<<1, 256, 0, s>>>(b.d_ctr, 1);
+        batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, 0, b.d_partials,
+                                                        b.dynamic ? b.d_ctr : nullptr);
+    }
+    if (b.dynamic) batch_zero_kernel<<<(reps + 255) / 256, 256, 0, s>>>(b.d_ctr, reps);
+    MSL_HIP_TRY(hipEventRecord(ctx().t0, s));
+    for (int k = 0; k < reps; k++)
+        batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, 0, b.d_partials,
+                                                        b.dynamic ? b.d_ctr + k : nullptr);
+    MSL_HIP_TRY(hipEventRecord(ctx().t1, s));
+    rc = check_launch("msl_batch_autotune");
+    if (rc) return rc;
+    MSL_HIP_TRY(hipEventSynchronize(ctx().t1));
+    float t = 0.0f;
+    MSL_HIP_TRY(hipEventElapsedTime(&t, ctx().t0, ctx().t1));
+    *ms = t / reps;
+    return MSL_OK;
+}
+
+static std::string device_tag() {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, ctx().device) != hipSuccess) return "unknown";
+    std::string n = p.gcnArchName;
+    for (char &ch : n)
+        if (ch == ' ') ch = '_';
+    return n + "/" + std::to_string(p.multiProcessorCount);
+}
+
+static bool load_table(Batch &b, const char *path, int *entries) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string magic, tag;
+    f >> magic >> tag;
+    if (magic != "msl-batch-tiles-v1" || tag != device_tag()) return false;
+    Tile t[kBatchMaxR + 1][kWidthClasses];
+    int n = 0, bpc = b.blocks_per_cu, dyn = b.dynamic;
+    std::string key;
+    while (f >> key) {
+        if (key == "tile") {
+            int r, wc, c, seg;
+            if (!(f >> r >> wc >> c >> seg)) return false;
+            if (r < 1 || r > kBatchMaxR || wc < 0 || wc >= kWidthClasses || (c != 1 && c != 4) || seg < 1) return false;
+            t[r][wc] = Tile{c, seg};
+            n++;
+        } else if (key == "launch") {
+            if (!(f >> bpc >> dyn)) return false;
+        } else {
+            return false;
+        }
+    }
+    std::memcpy(b.table, t, sizeof(t));
+    b.blocks_per_cu = bpc;
+    b.dynamic = dyn ? 1 : 0;
+    *entries = n;
+    return true;
+}
+
+static void save_table(const Batch &b, const char *path) {
+    std::ofstream f(path);
+    if (!f) return;
+    f << "msl-batch-tiles-v1 " << device_tag() << "\n";
+    for (int r = 1; r <= kBatchMaxR; r++)
+        for (int wc = 0; wc < kWidthClasses; wc++)
+            f << "tile " << r << " " << wc << " " << b.table[r][wc].c << " " << b.table[r][wc].seg << "\n";
+    f << "launch " << b.blocks_per_cu << " " << b.dynamic << "\n";
+}
+
+static int autotune(Batch &b, int *entries) {
+    static const Tile kCands[] = {{4, 16}, {4, 32}, {4, 64}, {4, 128}, {1, 16}, {1, 32}, {1, 64}, {1, 128}};
+    const int reps = 5;
+    int tuned = 0;
+    std::vector<std::vector<int>> members((kBatchMaxR + 1) * kWidthClasses);
+    for (int g = 0; g < (int)b.grids.size(); g++) {
+        const msl_batch_grid &G = b.grids[g];
+        if (G.rows > 0 && G.cols > 0) members[G.r * kWidthClasses + width_class(G.cols)].push_back(g);
+    }
+    const int saved_bpc = b.blocks_per_cu, saved_dyn = b.dynamic;
+    b.dynamic = 0;
+    for (int r = 1; r <= kBatchMaxR; r++) {
+        for (int wc = 0; wc < kWidthClasses; wc++) {
+            const std::vector<int> &pick = members[r * kWidthClasses + wc];
+            if (pick.empty()) continue;
+            float best = 1e30f;
+            Tile best_t = b.table[r][wc];
+            for (const Tile &cand : kCands) {
+                Tile trial[kBatchMaxR + 1][kWidthClasses];
+                std::memcpy(trial, b.table, sizeof(trial));
+                trial[r][wc] = cand;
+                std::vector<BGrid> hg;
+                std::vector<BItem> hi;
+                plan_items(b, pick, trial, hg, hi);
+                int rc = upload(b, hg, hi);
+                if (rc) return rc;
+                float ms = 0.0f;
+                rc = time_steps(b, reps, &ms);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_t = cand;
+                }
+            }
+            b.table[r][wc] = best_t;
+            tuned++;
+        }
+    }
+    // launch shape over the whole batch
+    int rc = rebuild(b);
+    if (rc) return rc;
+    float best = 1e30f;
+    int best_bpc = saved_bpc, best_dyn = saved_dyn;
+    for (int bpc = 1; bpc <= b.max_blocks_per_cu; bpc++) {
+        for (int dyn = 0; dyn <= 1; dyn++) {
+            b.blocks_per_cu = bpc;
+            b.dynamic = dyn;
+            float ms = 0.0f;
+            rc = time_steps(b, reps, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                best_bpc = bpc;
+                best_dyn = dyn;
+            }
+        }
+    }
+    b.blocks_per_cu = best_bpc;
+    b.dynamic = best_dyn;
+    *entries = tuned;
+    return MSL_OK;
+}
+
+static int check_batch(void *h, const char *what) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    if (!h) return fail(MSL_E_INVALID, std::string(what) + ": null batch");
+    if (ctx().capturing) return fail(MSL_E_STATE, std::string(what) + ": not allowed during graph capture");
+    return MSL_OK;
+}
+
+}  // namespace mslhip
+
+using namespace mslhip;
+
+extern "C" {
+
+int msl_batch_create(const msl_batch_grid *grids, int ngrids, void **h_batch) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    MSL_REQUIRE(h_batch != nullptr, "msl_batch_create: null out-pointer");
+    MSL_REQUIRE(ngrids >= 0 && (ngrids == 0 || grids != nullptr), "msl_batch_create: bad grid list");
+    if (ctx().capturing) return fail(MSL_E_STATE, "msl_batch_create: not allowed during graph capture");
+    for (int g = 0; g < ngrids; g++) {
+        const msl_batch_grid &G = grids[g];
+        MSL_REQUIRE(G.rows >= 0 && G.cols >= 0, "msl_batch_create: negative shape");
+        MSL_REQUIRE((int64_t)G.rows * G.cols <= INT32_MAX, "msl_batch_create: grid too large");
+        MSL_REQUIRE(G.r >= 1 && G.r <= kBatchMaxR, "msl_batch_create: radius must be in [1, 10]");
+        if (G.rows > 0 && G.cols > 0) {
+            MSL_REQUIRE(G.a && G.b && G.a != G.b, "msl_batch_create: null or aliased grid buffers");
+            MSL_REQUIRE((((uintptr_t)G.a | (uintptr_t)G.b) & 3u) == 0, "msl_batch_create: buffers must be 4-byte aligned");
+        }
+    }
+    Batch *b = new Batch();
+    b->grids.assign(grids, grids + ngrids);
+    b->blocks_per_cu = tune_int("MSL_BATCH_BLOCKS_PER_CU", 2);
+    b->dynamic = tune_int("MSL_BATCH_DYNAMIC", 0);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_step_kernel<true>, 256, 0) == hipSuccess && occ > 0)
+        b->max_blocks_per_cu = occ;
+    if (ngrids > 0) {
+        hipError_t e = hipMalloc(&b->d_grids, sizeof(BGrid) * ngrids);
+        if (e != hipSuccess) {
+            delete b;
+            return hip_fail(e, "msl_batch_create");
+        }
+    }
+    rc = rebuild(*b);
+    if (rc) {
+        msl_batch_destroy(b);
+        return rc;
+    }
+    *h_batch = b;
+    return MSL_OK;
+}
+
+int msl_batch_destroy(void *batch) {
+    if (!batch) return MSL_OK;
+    Batch *b = static_cast<Batch *>(batch);
+    if (ctx().ready) {
+        drop_graphs(*b);
+        (void)hipStreamSynchronize(ctx().stream);
+        if (b->d_grids) (void)hipFree(b->d_grids);
+        if (b->d_items) (void)hipFree(b->d_items);
+        if (b->d_partials) (void)hipFree(b->d_partials);
+        if (b->d_ctr) (void)hipFree(b->d_ctr);
+    }
+    delete b;
+    return MSL_OK;
+}
+
+int msl_batch_set_tile(void *batch, int r, int width_class_idx, int cols_per_lane, int seg) {
+    int rc = check_batch(batch, "msl_batch_set_tile");
+    if (rc) return rc;
+    MSL_REQUIRE(r >= 1 && r <= kBatchMaxR, "msl_batch_set_tile: radius must be in [1, 10]");
+    MSL_REQUIRE(width_class_idx >= 0 && width_class_idx < kWidthClasses, "msl_batch_set_tile: width class in [0, 3]");
+    MSL_REQUIRE(cols_per_lane == 1 || cols_per_lane == 4, "msl_batch_set_tile: cols_per_lane must be 1 or 4");
+    MSL_REQUIRE(seg >= 1, "msl_batch_set_tile: seg must be positive");
+    Batch *b = static_cast<Batch *>(batch);
+    b->table[r][width_class_idx] = Tile{cols_per_lane, seg};
+    return rebuild(*b);
+}
+
+int msl_batch_set_launch(void *batch, int blocks_per_cu, int dynamic_queue) {
+    int rc = check_batch(batch, "msl_batch_set_launch");
+    if (rc) return rc;
+    MSL_REQUIRE(blocks_per_cu >= 1, "msl_batch_set_launch: blocks_per_cu must be positive");
+    Batch *b = static_cast<Batch *>(batch);
+    drop_graphs(*b);
+    b->blocks_per_cu = blocks_per_cu;
+    b->dynamic = dynamic_queue ? 1 : 0;
+    return MSL_OK;
+}
+
+int msl_batch_autotune(void *batch, const char *cache_path, int *h_entries) {
+    int rc = check_batch(batch, "msl_batch_autotune");
+    if (rc) return rc;
+    Batch *b = static_cast<Batch *>(batch);
+    int entries = 0;
+    if (cache_path && load_table(*b, cache_path, &entries)) {
+        if (h_entries) *h_entries = -entries;  // negative: loaded from the cache
+        return rebuild(*b);
+    }
+    rc = autotune(*b, &entries);
+    if (rc) {
+        rebuild(*b);
+        return rc;
+    }
+    if (cache_path) save_table(*b, cache_path);
+    if (h_entries) *h_entries = entries;
+    return MSL_OK;
+}
+
+int msl_batch_info(const void *batch, int *h_items, int *h_blocks, int *h_blocks_per_cu, int *h_dynamic) {
+    MSL_REQUIRE(batch != nullptr, "msl_batch_info: null batch");
+    const Batch *b = static_cast<const Batch *>(batch);
+    if (h_items) *h_items = b->nitems;
+    if (h_blocks) *h_blocks = launch_blocks(*b);
+    if (h_blocks_per_cu) *h_blocks_per_cu = std::min(b->blocks_per_cu, b->max_blocks_per_cu);
+    if (h_dynamic) *h_dynamic = b->dynamic;
+    return MSL_OK;
+}
+
+int msl_batch_get_tile(const void *batch, int r, int width_class_idx, int *h_cols_per_lane, int *h_seg) {
+    MSL_REQUIRE(batch != nullptr, "msl_batch_get_tile: null batch");
+    MSL_REQUIRE(r >= 1 && r <= kBatchMaxR && width_class_idx >= 0 && width_class_idx < kWidthClasses,
+                "msl_batch_get_tile: bad class");
+    const Batch *b = static_cast<const Batch *>(batch);
+    if (h_cols_per_lane) *h_cols_per_lane = b->table[r][width_class_idx].c;
+    if (h_seg) *h_seg = b->table[r][width_class_idx].seg;
+    return MSL_OK;
+}
+
+int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b) {
+    int rc = check_batch(batch, "msl_batch_run");
+    if (rc) return rc;
+    MSL_REQUIRE(iters >= 0, "msl_batch_run: negative iteration count");
+    MSL_REQUIRE(iters >= 1 || d_sums == nullptr, "msl_batch_run: the fold needs at least one step");
+    Batch *b = static_cast<Batch *>(batch);
+    if (h_result_in_b) *h_result_in_b = iters & 1;
+    if (b->grids.empty()) return MSL_OK;
+    rc = ensure_ctr(*b, std::max(iters, 1));
+    if (rc) return rc;
+    const auto key = std::make_pair(iters, (const void *)d_sums);
+    auto it = b->graphs.find(key);
+    hipGraphExec_t exec = it == b->graphs.end() ? nullptr : it->second;
+    if (!exec) {
+        rc = msl_graph_begin();
+        if (rc) return rc;
+        int brc = enqueue_run(*b, iters, d_sums);
+        void *g = nullptr;
+        int erc = msl_graph_end(&g);
+        if (brc) {
+            if (g) msl_graph_destroy(g);
+            return brc;
+        }
+        if (erc) return erc;
+        exec = (hipGraphExec_t)g;
+        b->graphs[key] = exec;
+    }
+    MSL_HIP_TRY(hipGraphLaunch(exec, ctx().stream));
+    return MSL_OK;
+}
+
+int msl_batch_run_loop(void *batch, int iters, double *d_sums, int *h_result_in_b) {
+    int rc = check_batch(batch, "msl_batch_run_loop");
+    if (rc) return rc;
+    MSL_REQUIRE(iters >= 0, "msl_batch_run_loop: negative iteration count");
+    MSL_REQUIRE(iters >= 1 || d_sums == nullptr, "msl_batch_run_loop: the fold needs at least one step");
+    Batch *b = static_cast<Batch *>(batch);
+    if (h_result_in_b) *h_result_in_b = iters & 1;
+    for (int it = 0; it < iters; it++) {
+        for (const msl_batch_grid &G : b->grids) {
+            if (G.rows == 0 || G.cols == 0) continue;
+            float *src = (it & 1) ? G.b : G.a;
+            float *dst = (it & 1) ? G.a : G.b;
+            rc = mean_step(dst, src, G.rows, G.cols, G.r, G.nv, 0);
+            if (rc) return rc;
+        }
+    }
+    if (d_sums) {
+        const int ng = (int)b->grids.size();
+        if (ng > 0) {
+            batch_zero_f64_kernel<<<(ng + 255) / 256, 256, 0, ctx().stream>>>(d_sums, ng);
+            rc = check_launch("msl_batch_run_loop");
+            if (rc) return rc;
+        }
+        for (int g = 0; g < ng; g++) {
+            const msl_batch_grid &G = b->grids[g];
+            if (G.rows == 0 || G.cols == 0) continue;
+            const float *res = (iters & 1) ? G.b : G.a;
+            rc = msl_fold_sum_f32(res, (size_t)G.rows * G.cols, d_sums + g);
+            if (rc) return rc;
+        }
+    }
+    return MSL_OK;
+}
+
+}  // extern "C"

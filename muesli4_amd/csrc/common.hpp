@@ -71,6 +71,15 @@ __host__ __device__ inline uint64_t hash64(uint64_t seed, uint64_t idx) {
 __device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
 __device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
 
+// Knuth's TwoSum: s = RN(a + b) and the exact rounding error e, so that
+// a + b == s + e in real arithmetic.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = __fadd_rn(a, b);
+    float bp = __fsub_rn(s, a);
+    float ap = __fsub_rn(s, bp);
+    e = __fadd_rn(__fsub_rn(a, ap), __fsub_rn(b, bp));
+}
+
 // Workgroup barrier that orders LDS accesses only.  __syncthreads() also
 // waits for every outstanding global load of the wave, which would drain the
 // software prefetch queues of the stencil marches at each step; the

@@ -28,13 +28,6 @@ struct Pair {
     float s, c;
 };
 
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-    s = __fadd_rn(a, b);
-    float bp = __fsub_rn(s, a);
-    float ap = __fsub_rn(s, bp);
-    e = __fadd_rn(__fsub_rn(a, ap), __fsub_rn(b, bp));
-}
-
 // ---- policies --------------------------------------------------------------
 struct CompSumPolicy {
     using In = float;

@@ -1,0 +1,322 @@
+// Wavefront strip march for 2-D stencils of radius R (mean blur, Game of
+// Life), shared by the single-grid kernels and the batched persistent kernel.
+//
+// One wavefront owns a strip of 64*C columns (C adjacent columns per lane)
+// and a segment of rows, and marches down the segment:
+//   1. the input row y+R (strip plus R halo columns each side) is loaded,
+//      rows prefetched D ahead in registers, and staged in a per-wave LDS row;
+//   2. each lane reads its C+2R window back from LDS and forms C horizontal
+//      row sums (dx ascending, from zero);
+//   3. row sums live in a register ring of 2R+1 entries (the loop is unrolled
+//      over the ring period so every ring index is static) and each output is
+//      the ordered sum of its 2R+1 row sums (dy ascending, from zero),
+//      finished by the policy (mean: correctly rounded division by (2R+1)^2).
+// This is exactly the operation order of the scalar definition (bit-exact)
+// with 2(2R+1) adds per output instead of (2R+1)^2, one HBM read of each
+// input element (halo rows/columns come from L2) and one write per output.
+//
+// Row access modes:
+//   kAligned   - cols % C == 0 and 4C-byte aligned rows: a C-vector is wholly
+//                inside or outside the grid, one load per vector;
+//   kUnaligned - C = 4, any cols >= 4: the load is clamped into the row (a
+//                dword-aligned 16-byte load) and each column picks its value
+//                from the loaded quad by the clamping shift, so no lane
+//                branches;
+//   kScalar    - one clamped load per column (cols < C).
+#pragma once
+
+#include "common.hpp"
+
+namespace mslhip {
+
+enum RowMode { kAligned = 0, kUnaligned = 1, kScalar = 2 };
+
+template <typename T, int C>
+struct VecN {
+    T v[C];
+};
+
+template <typename T, int C>
+struct NativeVecN;
+template <typename T>
+struct NativeVecN<T, 1> {
+    typedef T type;
+};
+template <>
+struct NativeVecN<float, 4> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct NativeVecN<int32_t, 4> {
+    typedef int32_t type __attribute__((ext_vector_type(4)));
+};
+
+template <typename T, int C>
+__device__ __forceinline__ VecN<T, C> to_vecn(const typename NativeVecN<T, C>::type &x) {
+    VecN<T, C> r;
+    if constexpr (C == 1) {
+        r.v[0] = x;
+    } else {
+        r.v[0] = x.x;
+        r.v[1] = x.y;
+        r.v[2] = x.z;
+        r.v[3] = x.w;
+    }
+    return r;
+}
+
+template <typename T, int C>
+__device__ __forceinline__ typename NativeVecN<T, C>::type to_native(const VecN<T, C> &q) {
+    if constexpr (C == 1) {
+        return q.v[0];
+    } else {
+        return typename NativeVecN<T, C>::type{q.v[0], q.v[1], q.v[2], q.v[3]};
+    }
+}
+
+// Loads of C consecutive elements: p aligned to 4C bytes / to 4 bytes.
+template <typename T, int C>
+__device__ __forceinline__ VecN<T, C> load_aligned(const T *p) {
+    return to_vecn<T, C>(*reinterpret_cast<const typename NativeVecN<T, C>::type *>(p));
+}
+template <typename T, int C>
+__device__ __forceinline__ VecN<T, C> load_dword_aligned(const T *p) {
+    typename NativeVecN<T, C>::type x;
+    __builtin_memcpy(&x, p, sizeof(x));
+    return to_vecn<T, C>(x);
+}
+
+template <typename T, int C, int MODE>
+__device__ __forceinline__ VecN<T, C> read_vec(const T *__restrict__ in, int yy, int c, int rows, int cols, T nv) {
+    const int yc = min(max(yy, 0), rows - 1);
+    const bool rowok = yy == yc;
+    const T *rowp = in + (size_t)yc * cols;
+    VecN<T, C> q;
+    if constexpr (MODE == kAligned) {
+        const int cc = min(max(c, 0), cols - C);
+        const bool ok = rowok && cc == c;
+        const VecN<T, C> v = load_aligned<T, C>(rowp + cc);
+#pragma unroll
+        for (int k = 0; k < C; k++) q.v[k] = ok ? v.v[k] : nv;
+    } else if constexpr (MODE == kUnaligned) {
+        // [cc, cc+C) lies inside the row; column c+k sits at index k+s of the
+        // loaded vector whenever it is inside the grid.
+        const int cc = min(max(c, 0), cols - C);
+        const int s = c - cc;
+        const VecN<T, C> v = load_dword_aligned<T, C>(rowp + cc);
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int j = k + s;
+            T pick = v.v[0];
+#pragma unroll
+            for (int jj = 1; jj < C; jj++) pick = (j == jj) ? v.v[jj] : pick;
+            const int col = c + k;
+            q.v[k] = (rowok && col >= 0 && col < cols) ? pick : nv;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int ck = c + k;
+            const int cc = min(max(ck, 0), cols - 1);
+            const T v = rowp[cc];
+            q.v[k] = (rowok && cc == ck) ? v : nv;
+        }
+    }
+    return q;
+}
+
+// Stores the lane's C outputs of row orow starting at column x0.
+template <typename T, int C, int MODE>
+__device__ __forceinline__ void write_vec(T *__restrict__ orow, int x0, int cols, const VecN<T, C> &o) {
+    typedef typename NativeVecN<T, C>::type V;
+    if constexpr (MODE == kAligned) {
+        if (x0 < cols) __builtin_nontemporal_store(to_native<T, C>(o), reinterpret_cast<V *>(orow + x0));
+    } else if constexpr (MODE == kUnaligned) {
+        if (x0 + C <= cols) {
+            const V x = to_native<T, C>(o);
+            __builtin_memcpy(orow + x0, &x, sizeof(x));
+        } else {
+#pragma unroll
+            for (int k = 0; k < C; k++)
+                if (x0 + k < cols) orow[x0 + k] = o.v[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++)
+            if (x0 + k < cols) orow[x0 + k] = o.v[k];
+    }
+}
+
+// ---- per-stencil policies -------------------------------------------------
+template <int R_>
+struct MeanPolicy {
+    static constexpr int R = R_;
+    static constexpr bool kCenter = false;
+    typedef float T;
+    float denom;  // (2R+1)^2
+    float rcp;    // RN(1 / denom)
+    __device__ __host__ static MeanPolicy make() {
+        MeanPolicy p;
+        p.denom = (float)((2 * R + 1) * (2 * R + 1));
+        p.rcp = 1.0f / p.denom;
+        return p;
+    }
+    // Row sum of window w (index RP-R+c+d holds column x0+c+d-R).
+    template <int RP>
+    __device__ __forceinline__ float rowsum(const float *w, int c) const {
+        float s = 0.0f;
+#pragma unroll
+        for (int d = 0; d <= 2 * R; d++) s = __fadd_rn(s, w[RP - R + c + d]);
+        return s;
+    }
+    // Correctly rounded colsum / denom (identical to IEEE division).
+    __device__ __forceinline__ float finish(float colsum, float /*center*/) const {
+        return div_odd_square(colsum, denom, rcp);
+    }
+    __device__ __forceinline__ float colsum_init() const { return 0.0f; }
+    __device__ __forceinline__ float acc(float a, float b) const { return __fadd_rn(a, b); }
+};
+
+struct LifePolicy {
+    static constexpr int R = 1;
+    static constexpr bool kCenter = true;
+    typedef int32_t T;
+    float denom;  // unused
+    template <int RP>
+    __device__ __forceinline__ int32_t rowsum(const int32_t *w, int c) const {
+        return w[RP - 1 + c] + w[RP + c] + w[RP + 1 + c];
+    }
+    // B3/S23 as in examples/gameoflife.cpp:18-43 (sum of 8 neighbours).
+    __device__ __forceinline__ int32_t finish(int32_t box, int32_t live) const {
+        int32_t sum = box - live;
+        int32_t next = 0;
+        if (live == 1 && (sum == 2 || sum == 3)) next = 1;
+        if (live == 0 && sum == 3) next = 1;
+        return next;
+    }
+    __device__ __forceinline__ int32_t colsum_init() const { return 0; }
+    __device__ __forceinline__ int32_t acc(int32_t a, int32_t b) const { return a + b; }
+};
+
+template <int R, int C>
+struct StripConsts {
+    static_assert(C == 1 || C == 4, "1 or 4 columns per lane");
+    static constexpr int RP = (R + C - 1) / C * C;  // halo padded to whole vectors
+    static constexpr int SW = 64 * C;               // strip width
+    static constexpr int LDSW = SW + 2 * RP;        // elements per staged row
+    static constexpr int HL = RP / C;               // halo vectors per side
+    static constexpr int NQ = 2 * HL + 1;           // window vectors per lane
+    static constexpr int PER = 2 * R + 1;           // ring period (= loop unroll)
+    static_assert(2 * HL <= 64, "halo wider than a wavefront");
+    // Rows requested ahead of use: a divisor of PER (so queue slots stay
+    // static), larger where the per-row work is too short to hide latency.
+    // With one column per lane the rows are narrow and the whole period is
+    // kept in flight.
+    static constexpr int D = C == 1 ? PER
+                                    : (PER <= 7 ? PER : (R > 10 ? 1 : (PER % 5 == 0 ? 5 : (PER % 3 == 0 ? 3 : 1))));
+};
+
+// Sum of the values one wave stores, kept as a compensated (s, c) pair per
+// lane (TwoSum per element, in row order).
+struct LaneSum {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void push(float x) {
+        float t, e;
+        two_sum(s, x, t, e);
+        s = t;
+        c = __fadd_rn(c, e);
+    }
+};
+
+// March one wave down its (strip, segment): input rows y0-R .. yend-1+R,
+// output rows y0 .. yend-1 of the strip starting at column xs.  One code
+// path for every tile: reads are clamped (read_vec), stores are masked per
+// lane, and the loop is unrolled over exactly one ring period so the code
+// stays small (the instruction cache is shared by two CUs) while every
+// register index is a compile-time constant.  Row t+D is requested when row t
+// is staged (DQ > 0 overrides the default prefetch depth; it must divide
+// 2R+1).  With kSum, every stored value is also pushed into *sum.
+template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0>
+__device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
+                                            int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
+                                            const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW],
+                                            LaneSum *sum = nullptr) {
+    typedef typename Pol::T T;
+    typedef StripConsts<Pol::R, C> K;
+    constexpr int R = Pol::R;
+    constexpr int RP = K::RP;
+    constexpr int NQ = K::NQ;
+    constexpr int PER = K::PER;
+    constexpr int D = DQ > 0 ? DQ : K::D;
+    constexpr int HL = K::HL;
+    static_assert(PER % D == 0, "prefetch depth must divide the ring period");
+
+    const int lane = threadIdx.x & 63;
+    const int x0 = xs + C * lane;
+    const int total = (yend - y0) + 2 * R;  // input rows y0-R .. yend-1+R
+    // Halo vectors: left ones for lanes [0, HL), right ones for lanes
+    // [HL, 2HL); every other lane re-reads and re-stages its own centre
+    // vector, so no lane diverges.
+    const bool has_halo = lane < 2 * HL;
+    const int hx = has_halo ? (lane < HL ? xs - RP + C * lane : xs + K::SW + C * (lane - HL)) : x0;
+    const int hslot = has_halo ? (lane < HL ? C * lane : RP + K::SW + C * (lane - HL)) : RP + C * lane;
+
+    VecN<T, C> ring[PER];  // row sums
+    VecN<T, C> cring[Pol::kCenter ? PER : 1];
+    VecN<T, C> qc[D], qh[D];  // requested rows t .. t+D-1 (centre and halo vectors)
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        qc[d] = read_vec<T, C, MODE>(in, y0 - R + d, x0, rows, cols, nv);
+        qh[d] = read_vec<T, C, MODE>(in, y0 - R + d, hx, rows, cols, nv);
+    }
+
+    for (int tb = 0; tb < total; tb += PER) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int t = tb + u;  // ring slot of row t is u (tb is a multiple of PER)
+            const int qs = u % D;  // queue slot of row t
+            T *buf = lds[t & 1];
+            typedef typename NativeVecN<T, C>::type V;
+            *reinterpret_cast<V *>(buf + RP + C * lane) = to_native<T, C>(qc[qs]);
+            *reinterpret_cast<V *>(buf + hslot) = to_native<T, C>(qh[qs]);
+            // request row t+D into the slot just staged
+            qc[qs] = read_vec<T, C, MODE>(in, y0 - R + t + D, x0, rows, cols, nv);
+            qh[qs] = read_vec<T, C, MODE>(in, y0 - R + t + D, hx, rows, cols, nv);
+            wave_lds_sync();
+            T w[C * NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const VecN<T, C> v = to_vecn<T, C>(*reinterpret_cast<const V *>(buf + C * lane + C * q));
+#pragma unroll
+                for (int k = 0; k < C; k++) w[C * q + k] = v.v[k];
+            }
+#pragma unroll
+            for (int c = 0; c < C; c++) ring[u].v[c] = pol.template rowsum<RP>(w, c);
+            if constexpr (Pol::kCenter) {
+#pragma unroll
+                for (int c = 0; c < C; c++) cring[u].v[c] = w[RP + c];
+            }
+            const int y = y0 - 2 * R + t;  // output row completed by row t
+            if (t >= 2 * R && y < yend) {
+                VecN<T, C> o;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    T acc = pol.colsum_init();
+#pragma unroll
+                    for (int k = 0; k < PER; k++) acc = pol.acc(acc, ring[(u + 1 + k) % PER].v[c]);
+                    T centre = T(0);
+                    if constexpr (Pol::kCenter) centre = cring[(u + 1 + R) % PER].v[c];
+                    o.v[c] = pol.finish(acc, centre);
+                }
+                write_vec<T, C, MODE>(out + (size_t)y * cols, x0, cols, o);
+                if constexpr (kSum) {
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+                        if (x0 + c < cols) sum->push(o.v[c]);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace mslhip

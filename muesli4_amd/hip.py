@@ -36,6 +36,17 @@ PU32 = C.POINTER(C.c_uint32)
 PI64 = C.POINTER(C.c_int64)
 PP = C.POINTER(C.c_void_p)
 
+
+
+class BatchGrid(C.Structure):
+    """msl_batch_grid: one grid of a batched mean blur."""
+
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("rows", C.c_int), ("cols", C.c_int),
+                ("r", C.c_int), ("nv", C.c_float)]
+
+
+PBG = C.POINTER(BatchGrid)
+
 # name -> argtypes (all return int unless listed in _RESTYPES)
 SIGNATURES = {
     "msl_init": [i32],
@@ -75,6 +86,15 @@ SIGNATURES = {
     "msl_graph_end": [PP],
     "msl_graph_launch": [p],
     "msl_graph_destroy": [p],
+    "msl_batch_create": [PBG, i32, PP],
+    "msl_batch_destroy": [p],
+    "msl_batch_set_tile": [p, i32, i32, i32, i32],
+    "msl_batch_get_tile": [p, i32, i32, PI, PI],
+    "msl_batch_set_launch": [p, i32, i32],
+    "msl_batch_autotune": [p, C.c_char_p, PI],
+    "msl_batch_info": [p, PI, PI, PI, PI],
+    "msl_batch_run": [p, i32, p, PI],
+    "msl_batch_run_loop": [p, i32, p, PI],
     "msl_timer_start": [],
     "msl_timer_stop": [PF],
 }

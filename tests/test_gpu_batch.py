@@ -1,0 +1,186 @@
+"""GPU parity of the batched mean blur (msl_batch_*).
+
+Bar: every grid's final state bit-identical to the oracle's mean blur
+iterated on that grid alone (and to the plain one-launch-per-grid loop);
+per-grid fold(sum) within FOLD_REL_TOL of the FP64 sum of the final state.
+The same holds for every tile shape, launch shape and queue mode, and after
+autotuning.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from muesli4_amd import hip
+from muesli4_amd.batch import BatchArena
+from muesli4_amd.workloads import batch_shapes
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+# compensated FP32 per-item sums combined in FP64 vs the exact sum
+FOLD_REL_TOL = 1e-6
+
+
+def note(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    hip.init(0)
+    yield
+    hip.synchronize()
+
+
+def oracle_final(batch, g, iters):
+    x = R.fill_uniform_f32(batch.total, batch.seed)
+    grid = batch.grid(x, g)
+    for _ in range(iters):
+        grid = R.stencil2d_mean_f32(grid, int(batch.radii[g]), float(batch.nv[g]))
+    return grid
+
+
+def check_against_oracle(batch, iters, res_in_b, grids=None, oracle_init=None):
+    arena = batch.state("b" if res_in_b else "a")
+    init = R.fill_uniform_f32(batch.total, batch.seed) if oracle_init is None else oracle_init
+    sums = batch.fold_sums()
+    for g in (range(batch.n) if grids is None else grids):
+        want = batch.grid(init, g)
+        for _ in range(iters):
+            want = R.stencil2d_mean_f32(want, int(batch.radii[g]), float(batch.nv[g]))
+        got = batch.grid(arena, g)
+        np.testing.assert_array_equal(got, want, err_msg=f"grid {g} shape {want.shape} r={batch.radii[g]}")
+        ref = math.fsum(want.astype(np.float64).ravel())
+        assert abs(sums[g] - ref) <= FOLD_REL_TOL * max(abs(ref), 1e-30), (g, sums[g], ref)
+
+
+def small_batch(seed=5, n=40):
+    rows, cols, radii = batch_shapes(n, seed, r_max=10, side_min=8, side_max=300)
+    # edge shapes: a single row, a single column, cols < 4, radius larger
+    # than the grid, and an empty grid
+    rows = np.concatenate([rows, [1, 37, 5, 3, 0, 17]])
+    cols = np.concatenate([cols, [50, 1, 3, 2, 9, 0]])
+    radii = np.concatenate([radii, [3, 2, 10, 7, 1, 4]])
+    nv = np.where(np.arange(len(rows)) % 3 == 0, 0.5, 0.0).astype(np.float32)
+    return BatchArena(rows, cols, radii, nv=nv, seed=seed)
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_batch_matches_oracle(iters):
+    note(f"batch small iters={iters}")
+    b = small_batch()
+    res = b.run(iters)
+    hip.synchronize()
+    check_against_oracle(b, iters, res)
+    b.close()
+
+
+def test_batch_loop_baseline_matches_oracle():
+    note("batch loop baseline")
+    b = small_batch(seed=6)
+    res = b.run_loop(3)
+    hip.synchronize()
+    check_against_oracle(b, 3, res)
+    b.close()
+
+
+@pytest.mark.parametrize("cpl,seg", [(4, 16), (4, 128), (1, 16), (1, 64), (4, 7), (1, 1000)])
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_batch_tiles_and_queue_modes(cpl, seg, dynamic):
+    note(f"batch tiles cpl={cpl} seg={seg} dynamic={dynamic}")
+    b = small_batch(seed=7)
+    for r in range(1, 11):
+        for wc in range(4):
+            b.set_tile(r, wc, cpl, seg)
+    b.set_launch(1 if dynamic else 3, dynamic)
+    res = b.run(2)
+    hip.synchronize()
+    check_against_oracle(b, 2, res)
+    b.close()
+
+
+def test_batch_autotune_and_cache(tmp_path):
+    note("batch autotune")
+    b = small_batch(seed=8)
+    cache = str(tmp_path / "tiles.txt")
+    n = b.autotune(cache)
+    assert n > 0 and os.path.exists(cache)
+    res = b.run(2)
+    hip.synchronize()
+    check_against_oracle(b, 2, res)
+    b.close()
+    b2 = small_batch(seed=8)
+    n2 = b2.autotune(cache)
+    assert n2 < 0  # loaded from the cache
+    assert b2.info()["blocks_per_cu"] >= 1
+    b2.close()
+
+
+def test_batch_graph_replay_is_stable():
+    """Replaying the cached graph from the same initial state gives the same
+    bits every time (the queue counters are re-zeroed inside the graph)."""
+    note("batch graph replay")
+    b = small_batch(seed=9)
+    b.set_launch(2, True)
+    outs = []
+    for _ in range(3):
+        b.reset()
+        res = b.run(3)
+        hip.synchronize()
+        outs.append((b.state("b" if res else "a"), b.fold_sums()))
+    for arena, sums in outs[1:]:
+        np.testing.assert_array_equal(arena, outs[0][0])
+        np.testing.assert_array_equal(sums, outs[0][1])
+    b.close()
+
+
+def test_batch_rejects_bad_input():
+    import ctypes as C
+    from muesli4_amd.hip import BatchGrid, MslError, call
+
+    hip.init(0)
+    g = (BatchGrid * 1)()
+    g[0] = BatchGrid(None, None, 8, 8, 11, 0.0)
+    h = C.c_void_p()
+    with pytest.raises(MslError):
+        call("msl_batch_create", g, 1, C.byref(h))
+
+
+@pytest.mark.slow
+def test_batch_full_config_persistent_equals_loop():
+    """4096 grids (sides log-uniform in [32,1024], aspect up to 4:1,
+    r in 1..8), 10 steps plus fold: the persistent run and the plain loop
+    agree bit for bit on every grid; the smallest grids and two larger ones
+    are also checked against the oracle."""
+    note("batch full config 4096 grids x 10 steps")
+    rows, cols, radii = batch_shapes(4096, 2024)
+    b = BatchArena(rows, cols, radii, nv=0.0, seed=2024)
+    b.autotune(None)
+    res = b.run(10)
+    hip.synchronize()
+    fast = b.state("b" if res else "a")
+    fast_sums = b.fold_sums()
+    b.reset()
+    res2 = b.run_loop(10)
+    hip.synchronize()
+    assert res == res2
+    slow = b.state("b" if res2 else "a")
+    slow_sums = b.fold_sums()
+    for g in range(b.n):
+        np.testing.assert_array_equal(b.grid(fast, g), b.grid(slow, g), err_msg=f"grid {g}")
+    np.testing.assert_allclose(fast_sums, slow_sums, rtol=2 * FOLD_REL_TOL)
+    area = rows * cols
+    pick = list(np.argsort(area)[:12]) + [int(np.argsort(area)[len(area) // 2]), int(np.argmax(area))]
+    init = R.fill_uniform_f32(b.total, b.seed)
+    for g in pick:
+        want = b.grid(init, g)
+        for _ in range(10):
+            want = R.stencil2d_mean_f32(want, int(radii[g]), 0.0)
+        np.testing.assert_array_equal(b.grid(fast, g), want, err_msg=f"grid {g}")
+        ref = math.fsum(want.astype(np.float64).ravel())
+        assert abs(fast_sums[g] - ref) <= FOLD_REL_TOL * abs(ref)
+    b.close()
