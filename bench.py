@@ -33,6 +33,9 @@ SEED = 20240611
 BATCH_GRIDS = 4096
 BATCH_ITERS = 10
 BATCH_SEED = 2024
+SERVE_JOBS = 400
+SERVE_SEED = 2025
+SERVE_BUDGET = 64 << 30
 
 
 def log(msg):
@@ -188,6 +191,7 @@ def main():
         result["gol"] = bench_gol(hip, call, DeviceArray)
         result["heat3d"] = bench_heat(hip, call, DeviceArray)
         result["batched"] = bench_batch(hip)
+        result["serving"] = bench_serve(hip)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(value)
@@ -364,6 +368,53 @@ def bench_batch(hip):
            "tiles_cols_per_lane_seg_by_width_class": tiles}
     log(f"[bench] batched {out}")
     bt.close()
+    return out
+
+
+def bench_serve(hip):
+    """Serving: a seeded mix of 2-D mean-blur jobs (sides log-uniform in
+    [64,2048], r 1..10, 10..500 steps), 3-D heat jobs (side 32..256, 10..100
+    steps) and u32 map+fold jobs (length log-uniform in [1e4,1e8]).
+    Throughput: every job arrives at t=0 (saturation), continuous-batching
+    engine vs the plain loop (one job at a time, one launch + sync per
+    step).  Latency: Poisson arrivals at half of the plain loop's capacity,
+    served by both, and at half of the engine's own capacity."""
+    from muesli4_amd.serve import Server, job_traffic_bytes, latency_stats, plain_loop
+    from muesli4_amd.workloads import serve_trace
+
+    jobs, _ = serve_trace(SERVE_JOBS, SERVE_SEED, 0.0)
+    zeros = np.zeros(len(jobs))
+    srv = Server(SERVE_BUDGET, 1024)
+    srv.run_trace(jobs[:20], zeros[:20])  # warm-up: code objects, pinned buffers
+    res = srv.run_trace(jobs, zeros)
+    info = srv.info()
+    sat = latency_stats(res)
+    base = plain_loop(jobs, zeros, SERVE_BUDGET)
+    base_sat = latency_stats(base)
+    mismatches = sum(1 for a, b in zip(res, base)
+                     if (a["kind"] == 3 and a["sum_u32"] != b["sum_u32"])
+                     or (a["kind"] != 3 and abs(a["sum"] - b["sum"]) > 1e-5 * abs(b["sum"])))
+    traffic = sum(job_traffic_bytes(d) for d in jobs)
+    rate_low = 0.5 * base_sat["jobs_per_s"]
+    jobs_l, arr_l = serve_trace(SERVE_JOBS, SERVE_SEED, rate_low)
+    lat_srv_low = latency_stats(srv.run_trace(jobs_l, arr_l))
+    lat_base_low = latency_stats(plain_loop(jobs_l, arr_l, SERVE_BUDGET))
+    rate_high = 0.5 * sat["jobs_per_s"]
+    jobs_h, arr_h = serve_trace(SERVE_JOBS, SERVE_SEED, rate_high)
+    lat_srv_high = latency_stats(srv.run_trace(jobs_h, arr_h))
+    srv.close()
+    out = {"workload": f"{SERVE_JOBS} jobs: 50% 2-D mean blur (sides log-U[64,2048], r 1..10, 10..500 steps), "
+                       "25% 3-D heat7 (side 32..256, 10..100 steps), 25% u32 map(x*3+1)+fold "
+                       "(length log-U[1e4,1e8]); seeded",
+           "saturated": {"engine": sat, "plain_loop": base_sat,
+                         "speedup_jobs_per_s": round(sat["jobs_per_s"] / base_sat["jobs_per_s"], 2),
+                         "engine_GBs": round(traffic / sat["makespan_s"] / 1e9, 1),
+                         "result_mismatches_vs_plain_loop": mismatches},
+           "poisson_at_half_plain_capacity": {"rate_per_s": round(rate_low, 1), "engine": lat_srv_low,
+                                              "plain_loop": lat_base_low},
+           "poisson_at_half_engine_capacity": {"rate_per_s": round(rate_high, 1), "engine": lat_srv_high},
+           "engine_info": info, "hbm_budget_bytes": SERVE_BUDGET}
+    log(f"[bench] serving {out}")
     return out
 
 

@@ -7,7 +7,8 @@
  * reference skeleton or runtime function whose device work it replaces.
  *
  * Conventions
- *  - Every function returns int: MSL_OK (0) or a negative MSL_E_* code.
+ *  - Every function returns int: MSL_OK (0) or a negative MSL_E_* code
+ *    (msl_server_poll may also return MSL_NOT_READY).
  *    msl_last_error() returns a message for the calling thread's last error.
  *    Nothing throws across this boundary.  (Reference: errors print and
  *    continue, src/muesli.cpp:292-294; fatal paths exit(), src/ds.cpp:231.)
@@ -39,6 +40,7 @@ extern "C" {
 #define MSL_E_STATE (-4)     /* not initialised, or call not allowed now      */
 #define MSL_E_TIMEOUT (-5)   /* bounded wait expired                          */
 #define MSL_E_FULL (-6)      /* queue full (serving back-pressure)            */
+#define MSL_NOT_READY 1      /* not an error: the polled job is not done yet  */
 
 /* ------------------------------------------------------------------ */
 /* Runtime (reference: msl::initSkeletons / terminateSkeletons /
@@ -206,6 +208,82 @@ int msl_batch_autotune(void *batch, const char *cache_path, int *h_entries);
 int msl_batch_info(const void *batch, int *h_items, int *h_blocks, int *h_blocks_per_cu, int *h_dynamic);
 int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b);
 int msl_batch_run_loop(void *batch, int iters, double *d_sums, int *h_result_in_b);
+
+/* ------------------------------------------------------------------ */
+/* Serving engine.  No reference counterpart: a Muesli program serves one
+ * request at a time, building containers (src/ds.cpp:63-131), running each
+ * skeleton with a synchronisation after every call (src/muesli.cpp:213-223)
+ * and folding (src/ds.cpp:739-912); msl_serve_plain_loop keeps that model
+ * as the comparison baseline.
+ *
+ * Jobs (initial state generated on the device from `seed`):
+ *   MSL_JOB_BLUR2D      DM<float> d0 x d1 (rows, cols), values uniform in
+ *                       [0,255); `steps` mean-blur steps of radius r (1..10)
+ *                       with neutral value nv; result: compensated fold(sum)
+ *                       of the final state (sum).
+ *   MSL_JOB_HEAT3D      DC<float> d0 x d1 x d2 (nx, ny, nz), values uniform
+ *                       in [0,1); `steps` 7-point heat steps with neutral
+ *                       value nv; result: fold(sum) of the final state.
+ *   MSL_JOB_MAPFOLD_U32 DA<u32> of length n, bits from the seed;
+ *                       x <- x*mul + add (wrapping), then the wrapping
+ *                       fold(sum) (sum_u32).
+ * The engine admits queued jobs in arrival order while their HBM bytes fit
+ * its budget (one device allocation made at creation) and a slot is free,
+ * keeps each job's state resident, and every round advances all active
+ * jobs by one step with one persistent launch per job kind (continuous
+ * batching).  Final states and sums are bit-identical whether a job runs
+ * alone or in a mix.  Threading: msl_server_submit may be called from any
+ * thread; step / drain / poll / run_trace from one thread.              */
+#define MSL_JOB_BLUR2D 1
+#define MSL_JOB_HEAT3D 2
+#define MSL_JOB_MAPFOLD_U32 3
+
+typedef struct msl_job {
+    int kind;
+    int steps;         /* BLUR2D, HEAT3D: stencil steps (>= 1)             */
+    int r;             /* BLUR2D: radius 1..10                             */
+    int d0, d1, d2;    /* BLUR2D: rows, cols; HEAT3D: nx, ny, nz           */
+    int64_t n;         /* MAPFOLD_U32: length                              */
+    float nv;          /* stencils: neutral value outside the grid         */
+    uint32_t mul, add; /* MAPFOLD_U32: x <- x*mul + add                    */
+    uint64_t seed;     /* initial state                                    */
+    int keep_final;    /* keep the final state until msl_server_poll      */
+} msl_job;
+
+typedef struct msl_job_result {
+    int64_t id;
+    int kind;
+    double sum;        /* BLUR2D, HEAT3D                                   */
+    uint32_t sum_u32;  /* MAPFOLD_U32                                      */
+    double t_arrival;  /* seconds (server clock, or since the trace start) */
+    double t_admit;
+    double t_done;     /* when the host saw the result                     */
+    size_t bytes;      /* HBM held while active                            */
+} msl_job_result;
+
+int msl_server_create(size_t hbm_budget_bytes, int max_active, void **h_server);
+int msl_server_destroy(void *server);
+/* t_arrival < 0: now.  MSL_E_NOMEM if the job can never fit the budget. */
+int msl_server_submit(void *server, const msl_job *job, double t_arrival, int64_t *h_id);
+/* One round (admit, advance every active job one step, retire finished
+ * rounds); *h_active / *h_pending after the round.                     */
+int msl_server_step(void *server, int *h_active, int *h_pending);
+/* Rounds until nothing is pending, active or in flight.                */
+int msl_server_drain(void *server);
+/* MSL_OK and the result once the job is done (its record is then
+ * released), MSL_NOT_READY while it is not.  h_final (nullable) receives
+ * the final state of a job submitted with keep_final.                  */
+int msl_server_poll(void *server, int64_t id, msl_job_result *h_result, void *h_final, size_t final_bytes);
+int msl_server_info(const void *server, int64_t *h_rounds, size_t *h_peak_bytes, double *h_mean_active,
+                    int64_t *h_done_jobs);
+/* Replays an arrival trace in real time (arrival_s ascending, seconds from
+ * the call) on an idle server and returns every job's result.         */
+int msl_server_run_trace(void *server, const msl_job *jobs, const double *arrival_s, int njobs,
+                         msl_job_result *h_results);
+/* Baseline: the same trace, one job at a time, one launch plus one stream
+ * synchronisation per step, then a fold copied to the host.            */
+int msl_serve_plain_loop(const msl_job *jobs, const double *arrival_s, int njobs, size_t hbm_budget_bytes,
+                         msl_job_result *h_results);
 
 /* ------------------------------------------------------------------ */
 /* Event timing on the library stream (for benchmarks). */

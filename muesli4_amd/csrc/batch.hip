@@ -40,18 +40,11 @@
 #include <string>
 #include <vector>
 
-#include "strip.hpp"
+#include "blur_item.hpp"
 
 namespace mslhip {
 
 int mean_step(float *out, const float *in, int rows, int cols, int r, float nv, int seg);
-
-constexpr int kBatchMaxR = 10;
-constexpr int kBatchMaxR4 = 8;  // largest radius run with 4 columns per lane
-constexpr int kWidthClasses = 4;
-constexpr int kKindAligned4 = 0;    // C = 4, aligned rows
-constexpr int kKindUnaligned4 = 1;  // C = 4, dword-aligned rows
-constexpr int kKindNarrow1 = 2;     // C = 1
 
 struct BGrid {
     float *a;
@@ -64,50 +57,6 @@ struct BGrid {
 struct BItem {
     int grid, xs, y0, y1;
 };
-
-// Prefetch depth per (R, C) in the batched kernel: divisors of 2R+1 chosen
-// so that the largest specialisation stays within 128 VGPRs (four
-// workgroups per CU); narrow strips keep the whole ring period in flight.
-template <int R, int C>
-struct BatchDepth {
-    static constexpr int PER = 2 * R + 1;
-    static constexpr int D = C == 1 ? PER : (PER <= 5 ? PER : (PER % 3 == 0 ? 3 : 1));
-};
-
-inline int width_class(int cols) {
-    if (cols < 96) return 0;
-    if (cols < 192) return 1;
-    if (cols < 384) return 2;
-    return 3;
-}
-
-template <int R, bool kFold>
-__device__ __forceinline__ void batch_item(const BGrid &g, const BItem &it, int parity, float *ldsw, LaneSum *ls) {
-    const float *src = parity ? g.b : g.a;
-    float *dst = parity ? g.a : g.b;
-    const MeanPolicy<R> pol = MeanPolicy<R>::make();
-    if constexpr (R > kBatchMaxR4) {
-        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
-            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
-            reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls);
-    } else if (g.kind == kKindAligned4) {
-        strip_march<MeanPolicy<R>, 4, kAligned, kFold, BatchDepth<R, 4>::D>(
-            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
-            reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls);
-    } else if (g.kind == kKindUnaligned4) {
-        strip_march<MeanPolicy<R>, 4, kUnaligned, kFold, BatchDepth<R, 4>::D>(
-            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
-            reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls);
-    } else {
-        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
-            dst, src, g.rows, g.cols, g.nv, it.xs, it.y0, it.y1, pol,
-            reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls);
-    }
-}
-
-constexpr int kBatchLdsw = StripConsts<kBatchMaxR4, 4>::LDSW > StripConsts<kBatchMaxR, 1>::LDSW
-                                 ? StripConsts<kBatchMaxR4, 4>::LDSW
-                                 : StripConsts<kBatchMaxR, 1>::LDSW;
 
 template <bool kFold>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict__ grids,
@@ -122,28 +71,11 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict
         const BItem it = items[i];
         const BGrid g = grids[it.grid];
         LaneSum ls;
-        float *ldsw = lds[wave];
-        switch (g.r) {
-            case 1: batch_item<1, kFold>(g, it, parity, ldsw, &ls); break;
-            case 2: batch_item<2, kFold>(g, it, parity, ldsw, &ls); break;
-            case 3: batch_item<3, kFold>(g, it, parity, ldsw, &ls); break;
-            case 4: batch_item<4, kFold>(g, it, parity, ldsw, &ls); break;
-            case 5: batch_item<5, kFold>(g, it, parity, ldsw, &ls); break;
-            case 6: batch_item<6, kFold>(g, it, parity, ldsw, &ls); break;
-            case 7: batch_item<7, kFold>(g, it, parity, ldsw, &ls); break;
-            case 8: batch_item<8, kFold>(g, it, parity, ldsw, &ls); break;
-            case 9: batch_item<9, kFold>(g, it, parity, ldsw, &ls); break;
-            default: batch_item<10, kFold>(g, it, parity, ldsw, &ls); break;
-        }
+        const float *src = parity ? g.b : g.a;
+        float *dst = parity ? g.a : g.b;
+        blur_item_r<kFold>(g.r, src, dst, g.rows, g.cols, g.nv, g.kind, it.xs, it.y0, it.y1, lds[wave], &ls);
         if constexpr (kFold) {
-#pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) {
-                const float os = __shfl_xor(ls.s, m), oc = __shfl_xor(ls.c, m);
-                float s, e;
-                two_sum(ls.s, os, s, e);
-                ls.c = __fadd_rn(__fadd_rn(ls.c, oc), e);
-                ls.s = s;
-            }
+            wave_pair_reduce(ls);
             if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
         }
         if (ctr) {

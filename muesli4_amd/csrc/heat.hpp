@@ -1,0 +1,102 @@
+// 7-point heat march of one wavefront over a 64 x 8 column tile and a
+// segment of z-planes, shared by the single-cube kernel and the batched
+// serving kernel.
+//
+// Lane l owns column x = x0 + l and keeps, for a ring of kP planes, the kTY
+// cells of its column plus the two y-halo cells (rows y0-1 and y0+kTY) and
+// one x-halo cell per row (lane 0 reads x0-1, lane 63 reads x0+64, every
+// other lane re-reads its own cell).  x neighbours come from the adjacent
+// lanes by DPP, y neighbours from the lane's own registers, z neighbours from
+// the plane ring; each plane is requested one step before its use.  Each cell
+//   v = 0 + in(x-1) + in(x+1) + in(y-1) + in(y+1) + in(z-1) + in(z+1); v / 6
+// is computed in exactly that order (examples/heatdiffusion.cpp:55-65).
+#pragma once
+
+#include "strip.hpp"
+
+namespace mslhip {
+
+constexpr int kTX = 64;  // x extent of a wave tile (one cell per lane)
+constexpr int kTY = 8;   // y extent of a wave tile (cells per lane)
+constexpr int kP = 4;    // plane ring: z-1, z, z+1 and the requested z+2
+
+// Branch-free cell read: the address is clamped into the cube and the value
+// replaced by nv when the cell lies outside, so loads are never predicated
+// and the compiler's memory-counter waits stay exact across the march.
+__device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, int y, int z, int nx, int ny, int nz,
+                                         float nv) {
+    const int xc = min(max(x, 0), nx - 1);
+    const int yc = min(max(y, 0), ny - 1);
+    const int zc = min(max(z, 0), nz - 1);
+    const float v = in[((size_t)zc * ny + yc) * nx + xc];
+    return (x == xc && y == yc && z == zc) ? v : nv;
+}
+
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane
+// i-1 (resp. i+1); the lane whose source is outside the wave keeps 'edge'.
+__device__ __forceinline__ float from_left(float v, float edge) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v, float edge) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+// March the tile (x0 = 64 * xtile, rows y0 .. y0+kTY-1) over planes
+// z0 .. z1-1.  kResidual: fold max |v - in| into *dmax.  kSum: push every
+// stored value into *sum.
+template <bool kResidual, bool kSum>
+__device__ __forceinline__ void heat7_march(float *__restrict__ out, const float *__restrict__ in, int nx, int ny,
+                                            int nz, float nv, int x0, int y0, int z0, int z1, float *dmax,
+                                            LaneSum *sum) {
+    const int lane = threadIdx.x & 63;
+    const int x = x0 + lane;
+    const int hxl = lane == 0 ? x - 1 : (lane == 63 ? x + 1 : x);  // x-halo column
+
+    // slot(zz) = (zz - z0 + 1) & (kP-1); col[j] is row y0-1+j
+    float col[kTY + 2][kP];
+    float hal[kTY][kP];
+    auto load_plane = [&](int zz, int slot) {
+#pragma unroll
+        for (int j = 0; j < kTY + 2; j++) col[j][slot] = ld_cell(in, x, y0 - 1 + j, zz, nx, ny, nz, nv);
+#pragma unroll
+        for (int j = 0; j < kTY; j++) hal[j][slot] = ld_cell(in, hxl, y0 + j, zz, nx, ny, nz, nv);
+    };
+#pragma unroll
+    for (int k = 0; k < kP - 1; k++) load_plane(z0 - 1 + k, k);
+
+    // Whole blocks of kP planes; planes past z1 are computed, not stored.
+    for (int zb = z0; zb < z1; zb += kP) {
+#pragma unroll
+        for (int u = 0; u < kP; u++) {
+            const int z = zb + u;
+            const int sb = u & (kP - 1);        // z-1
+            const int sc = (u + 1) & (kP - 1);  // z
+            const int sa = (u + 2) & (kP - 1);  // z+1
+            // request plane z+2 into the slot of z-2 (free since step z-1)
+            load_plane(z + 2, (u + 3) & (kP - 1));
+#pragma unroll
+            for (int j = 0; j < kTY; j++) {
+                const float c = col[j + 1][sc];
+                float v = 0.0f;
+                v = __fadd_rn(v, from_left(c, hal[j][sc]));   // x-1
+                v = __fadd_rn(v, from_right(c, hal[j][sc]));  // x+1
+                v = __fadd_rn(v, col[j][sc]);                 // y-1
+                v = __fadd_rn(v, col[j + 2][sc]);             // y+1
+                v = __fadd_rn(v, col[j + 1][sb]);             // z-1
+                v = __fadd_rn(v, col[j + 1][sa]);             // z+1
+                v = __fdiv_rn(v, 6.0f);
+                const int y = y0 + j;
+                if (z < z1 && x < nx && y < ny) {
+                    __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
+                    if constexpr (kResidual) {
+                        const float d = fabsf(__fsub_rn(v, c));
+                        *dmax = d > *dmax ? d : *dmax;
+                    }
+                    if constexpr (kSum) sum->push(v);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace mslhip

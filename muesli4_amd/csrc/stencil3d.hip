@@ -6,54 +6,21 @@
 // functor examples/heatdiffusion.cpp:55-65).
 //
 // Here one wavefront owns a 64 x 8 column of the cube and marches a segment
-// of z-planes (2.5-D blocking) with all neighbours in registers: y and z
-// neighbours in the lane's own plane ring, x neighbours from the adjacent
-// lanes by DPP lane shifts; each plane is requested one step before its use.  Each cell
-//   v = 0 + in(x-1) + in(x+1) + in(y-1) + in(y+1) + in(z-1) + in(z+1); v / 6
-// is computed in exactly that order.  Each cell is read from HBM once and written once
-// (8 bytes per cell per step); the optional fused residual folds
+// of z-planes with all neighbours in registers (heat7_march): each cell is
+// read from HBM once and written once (8 bytes per cell per step), in the
+// reference functor's operation order.  The optional fused residual folds
 // max |v - in| into one device scalar with an integer atomic max (the bit
 // patterns of non-negative floats order like unsigned integers).
 #include <map>
 #include <mutex>
 #include <string>
 
-#include "common.hpp"
+#include "heat.hpp"
 
 namespace mslhip {
 
-constexpr int kTX = 64;  // x extent of a wave tile (one cell per lane)
-constexpr int kTY = 8;   // y extent of a wave tile (cells per lane)
-constexpr int kP = 4;    // plane ring: z-1, z, z+1 and the requested z+2
-
-// Branch-free cell read: the address is clamped into the cube and the value
-// replaced by nv when the cell lies outside, so loads are never predicated
-// and the compiler's memory-counter waits stay exact across the march.
-__device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, int y, int z, int nx, int ny, int nz,
-                                         float nv) {
-    const int xc = min(max(x, 0), nx - 1);
-    const int yc = min(max(y, 0), ny - 1);
-    const int zc = min(max(z, 0), nz - 1);
-    const float v = in[((size_t)zc * ny + yc) * nx + xc];
-    return (x == xc && y == yc && z == zc) ? v : nv;
-}
-
-// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane
-// i-1 (resp. i+1); the lane whose source is outside the wave keeps 'edge'.
-__device__ __forceinline__ float from_left(float v, float edge) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float from_right(float v, float edge) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
-}
-
 // One wave per (64 x kTY column tile, z-segment); four independent waves per
-// workgroup, no LDS and no barrier.  Lane l owns column x = x0 + l and keeps,
-// for a ring of kP planes, the kTY cells of its column plus the two y-halo
-// cells (rows y0-1 and y0+kTY) and one x-halo cell per row (lane 0 reads
-// x0-1, lane 63 reads x0+64, every other lane re-reads its own cell).  x
-// neighbours come from the adjacent lanes by DPP, y neighbours from the
-// lane's own registers, z neighbours from the plane ring.
+// workgroup, no LDS and no barrier.
 template <bool kResidual>
 __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, const float *__restrict__ in, int nx,
                                                     int ny, int nz, float nv, int zseg, int xtiles, int ytiles,
@@ -66,56 +33,9 @@ __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, con
         const int rest = item / xtiles;
         const int yt = rest % ytiles;
         const int zs = rest / ytiles;
-        const int x = xt * kTX + lane;
-        const int y0 = yt * kTY;
         const int z0 = zs * zseg;
-        const int z1 = min(z0 + zseg, nz);
-        const int hxl = lane == 0 ? x - 1 : (lane == 63 ? x + 1 : x);  // x-halo column
-
-        // slot(zz) = (zz - z0 + 1) & (kP-1); col[j] is row y0-1+j
-        float col[kTY + 2][kP];
-        float hal[kTY][kP];
-        auto load_plane = [&](int zz, int slot) {
-#pragma unroll
-            for (int j = 0; j < kTY + 2; j++) col[j][slot] = ld_cell(in, x, y0 - 1 + j, zz, nx, ny, nz, nv);
-#pragma unroll
-            for (int j = 0; j < kTY; j++) hal[j][slot] = ld_cell(in, hxl, y0 + j, zz, nx, ny, nz, nv);
-        };
-#pragma unroll
-        for (int k = 0; k < kP - 1; k++) load_plane(z0 - 1 + k, k);
-
-        // Whole blocks of kP planes; planes past z1 are computed, not stored.
-        for (int zb = z0; zb < z1; zb += kP) {
-#pragma unroll
-            for (int u = 0; u < kP; u++) {
-                const int z = zb + u;
-                const int sb = u & (kP - 1);        // z-1
-                const int sc = (u + 1) & (kP - 1);  // z
-                const int sa = (u + 2) & (kP - 1);  // z+1
-                // request plane z+2 into the slot of z-2 (free since step z-1)
-                load_plane(z + 2, (u + 3) & (kP - 1));
-#pragma unroll
-                for (int j = 0; j < kTY; j++) {
-                    const float c = col[j + 1][sc];
-                    float v = 0.0f;
-                    v = __fadd_rn(v, from_left(c, hal[j][sc]));   // x-1
-                    v = __fadd_rn(v, from_right(c, hal[j][sc]));  // x+1
-                    v = __fadd_rn(v, col[j][sc]);                 // y-1
-                    v = __fadd_rn(v, col[j + 2][sc]);             // y+1
-                    v = __fadd_rn(v, col[j + 1][sb]);             // z-1
-                    v = __fadd_rn(v, col[j + 1][sa]);             // z+1
-                    v = __fdiv_rn(v, 6.0f);
-                    const int y = y0 + j;
-                    if (z < z1 && x < nx && y < ny) {
-                        __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
-                        if (kResidual) {
-                            const float d = fabsf(__fsub_rn(v, c));
-                            dmax = d > dmax ? d : dmax;
-                        }
-                    }
-                }
-            }
-        }
+        heat7_march<kResidual, false>(out, in, nx, ny, nz, nv, xt * kTX, yt * kTY, z0, min(z0 + zseg, nz), &dmax,
+                                      nullptr);
     }
     if (kResidual) {
         __shared__ float wmax[4];

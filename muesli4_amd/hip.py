@@ -47,6 +47,30 @@ class BatchGrid(C.Structure):
 
 PBG = C.POINTER(BatchGrid)
 
+MSL_JOB_BLUR2D = 1
+MSL_JOB_HEAT3D = 2
+MSL_JOB_MAPFOLD_U32 = 3
+
+
+class Job(C.Structure):
+    """msl_job: one serving request."""
+
+    _fields_ = [("kind", C.c_int), ("steps", C.c_int), ("r", C.c_int), ("d0", C.c_int), ("d1", C.c_int),
+                ("d2", C.c_int), ("n", C.c_int64), ("nv", C.c_float), ("mul", C.c_uint32), ("add", C.c_uint32),
+                ("seed", C.c_uint64), ("keep_final", C.c_int)]
+
+
+class JobResult(C.Structure):
+    """msl_job_result."""
+
+    _fields_ = [("id", C.c_int64), ("kind", C.c_int), ("sum", C.c_double), ("sum_u32", C.c_uint32),
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("bytes", C.c_size_t)]
+
+
+PJOB = C.POINTER(Job)
+PRES = C.POINTER(JobResult)
+
 # name -> argtypes (all return int unless listed in _RESTYPES)
 SIGNATURES = {
     "msl_init": [i32],
@@ -95,6 +119,15 @@ SIGNATURES = {
     "msl_batch_info": [p, PI, PI, PI, PI],
     "msl_batch_run": [p, i32, p, PI],
     "msl_batch_run_loop": [p, i32, p, PI],
+    "msl_server_create": [sz, i32, PP],
+    "msl_server_destroy": [p],
+    "msl_server_submit": [p, PJOB, C.c_double, PI64],
+    "msl_server_step": [p, PI, PI],
+    "msl_server_drain": [p],
+    "msl_server_poll": [p, C.c_int64, PRES, p, sz],
+    "msl_server_info": [p, PI64, C.POINTER(C.c_size_t), PD, PI64],
+    "msl_server_run_trace": [p, PJOB, PD, i32, PRES],
+    "msl_serve_plain_loop": [PJOB, PD, i32, sz, PRES],
     "msl_timer_start": [],
     "msl_timer_stop": [PF],
 }
