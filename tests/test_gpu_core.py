@@ -337,6 +337,56 @@ def test_heat7_iterate_with_residuals():
         assert list(res) == ref_res
 
 
+# Drift of the FP32 stencils against FP64 recurrences on the same inputs
+# (reported; bounded by DRIFT_ABS_TOL).  The FP32 results themselves are
+# bit-exact against the FP32 oracle.
+DRIFT_ABS_TOL = 1e-5
+
+
+def test_heat7_drift_vs_fp64():
+    shape = (64, 64, 64)
+    nz, ny, nx = shape
+    iters = 200
+    note("heat7 drift 64^3 x 200")
+    n = nx * ny * nz
+    a = dev_uniform(n, 43, R.SCALE_UNIT)
+    b = DeviceArray(shape, np.float32)
+    inb = C.c_int()
+    call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, 0, None, C.byref(inb))
+    got = (b if inb.value else a).to_host().reshape(shape)
+    x32 = R.fill_uniform_f32(n, 43, R.SCALE_UNIT).reshape(shape)
+    x64 = x32.astype(np.float64)
+    drift = []
+    for it in range(iters):
+        x32 = R.heat7_f32(x32, 0.0)
+        x64 = R.heat7_f64(x64, 0.0)
+        if (it + 1) % 50 == 0:
+            drift.append(float(np.abs(x32.astype(np.float64) - x64).max()))
+    np.testing.assert_array_equal(got, x32)
+    rel = float((np.abs(got.astype(np.float64) - x64) / np.maximum(np.abs(x64), 1e-30)).max())
+    print(f"[drift] heat7 64^3 max|f32-f64| every 50 steps {drift}, max rel after {iters}: {rel:.3e}")
+    assert max(drift) < DRIFT_ABS_TOL
+
+
+def test_mean_blur_drift_vs_fp64():
+    rows, cols, r, iters = 256, 320, 5, 20
+    note("mean blur drift 256x320 r5 x 20")
+    a = dev_uniform(rows * cols, 44)
+    b = DeviceArray((rows, cols), np.float32)
+    inb = C.c_int()
+    call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, 0.0, iters, C.byref(inb))
+    got = (b if inb.value else a).to_host().reshape(rows, cols)
+    x32 = R.fill_uniform_f32(rows * cols, 44).reshape(rows, cols)
+    x64 = x32.astype(np.float64)
+    for _ in range(iters):
+        x32 = R.stencil2d_mean_f32(x32, r, 0.0)
+        x64 = R.stencil2d_mean_f64(x64, r, 0.0)
+    np.testing.assert_array_equal(got, x32)
+    d = float(np.abs(got.astype(np.float64) - x64).max())
+    print(f"[drift] mean blur r={r} after {iters} steps: max|f32-f64| = {d:.3e} (values < 255)")
+    assert d < 255 * 1e-5
+
+
 # ---- graph capture of arbitrary calls ------------------------------------------
 def test_graph_capture_replays():
     note("graph capture")

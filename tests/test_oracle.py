@@ -228,3 +228,43 @@ def test_boxsum_all_ones_closed_form():
     assert out[5, 5, 5] == 27 and out[0, 5, 5] == 18 and out[0, 0, 5] == 12 and out[0, 0, 0] == 8
     vals, counts = np.unique(out, return_counts=True)
     assert set(vals.tolist()) == {8.0, 12.0, 18.0, 27.0}
+
+
+# ---- FP64 drift references -------------------------------------------------------
+def test_mean_blur_f64_brute_force():
+    a = R.fill_uniform_f32(7 * 10, 31).reshape(7, 10).astype(np.float64)
+    r, nv = 2, 0.75
+    rows, cols = a.shape
+    expect = np.empty_like(a)
+    for y in range(rows):
+        for x in range(cols):
+            tot = 0.0
+            for dy in range(-r, r + 1):
+                row = 0.0
+                for dx in range(-r, r + 1):
+                    yy, xx = y + dy, x + dx
+                    row += a[yy, xx] if (0 <= yy < rows and 0 <= xx < cols) else nv
+                tot += row
+            expect[y, x] = tot / (2 * r + 1) ** 2
+    np.testing.assert_array_equal(R.stencil2d_mean_f64(a, r, nv), expect)
+
+
+def test_heat7_f64_brute_force_and_closed_form():
+    nz, ny, nx = 4, 5, 6
+    a = R.fill_uniform_f32(nz * ny * nx, 32, R.SCALE_UNIT).reshape(nz, ny, nx).astype(np.float64)
+
+    def g(z, y, x):
+        return a[z, y, x] if (0 <= z < nz and 0 <= y < ny and 0 <= x < nx) else 0.0
+
+    expect = np.empty_like(a)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                v = 0.0
+                for (dz, dy, dx) in [(0, 0, -1), (0, 0, 1), (0, -1, 0), (0, 1, 0), (-1, 0, 0), (1, 0, 0)]:
+                    v += g(z + dz, y + dy, x + dx)
+                expect[z, y, x] = v / 6.0
+    np.testing.assert_array_equal(R.heat7_f64(a, 0.0), expect)
+    ones = np.ones((5, 5, 5))
+    out = R.heat7_f64(ones, 0.0)
+    assert out[2, 2, 2] == 1.0 and out[0, 0, 0] == 0.5
