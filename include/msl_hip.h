@@ -197,8 +197,9 @@ int msl_batch_destroy(void *batch);
 int msl_batch_set_tile(void *batch, int r, int width_class, int cols_per_lane, int seg);
 int msl_batch_get_tile(const void *batch, int r, int width_class, int *h_cols_per_lane, int *h_seg);
 /* Persistent launch shape: workgroups per CU (capped by occupancy) and
- * static round-robin (0) or atomic work queue (1).                    */
-int msl_batch_set_launch(void *batch, int blocks_per_cu, int dynamic_queue);
+ * queue mode: 0 static round-robin, 1 one atomic queue per XCD over a
+ * cost-balanced item range, 2 one global atomic queue.                 */
+int msl_batch_set_launch(void *batch, int blocks_per_cu, int queue_mode);
 /* Autotuner: times every tile candidate per (radius, width class) present
  * in the batch, then the launch shape, and keeps the fastest.  With
  * cache_path, a table saved for this device is loaded instead (then

@@ -84,8 +84,10 @@ class BatchArena:
         call("msl_batch_get_tile", self.handle, r, width_class, C.byref(c), C.byref(s))
         return c.value, s.value
 
-    def set_launch(self, blocks_per_cu: int, dynamic: bool) -> None:
-        call("msl_batch_set_launch", self.handle, blocks_per_cu, 1 if dynamic else 0)
+    def set_launch(self, blocks_per_cu: int, queue_mode) -> None:
+        """queue_mode: 0 static, 1 per-XCD atomic queues, 2 one global queue
+        (True/False map to 1/0)."""
+        call("msl_batch_set_launch", self.handle, blocks_per_cu, int(queue_mode))
 
     def info(self) -> dict:
         v = [C.c_int() for _ in range(4)]

@@ -17,20 +17,20 @@
 // time execute the same specialisation (the instruction cache is shared by
 // two CUs).  The persistent launch holds `blocks_per_cu` 4-wave workgroups
 // per CU; wave w takes items w, w+W, ... (static) or, with the dynamic
-// queue, its first item statically and every further item from an atomic
-// counter.  Consecutive items map to waves of one XCD (xcd_tile), so the
-// halo rows shared by neighbouring segments are served by one L2.
+// queue, the items are split into one cost-balanced contiguous range per
+// XCD and the XCD's waves take them from that range's atomic counter.
+// Either way consecutive items run on one XCD, so the halo rows shared by
+// neighbouring segments are served by one L2.
 //
 // Fold.  On the last step every lane keeps a TwoSum-compensated sum of the
 // values it stores; a wave combines its lanes in a fixed butterfly and
 // writes one (s, c) pair per item; a final kernel adds a grid's pairs in
 // FP64 in item order.  Deterministic for a given tiling.
 //
-// Autotuner.  Tile shape (C, seg) is chosen per (radius, width class) by
-// timing one step over the sub-batch of that class for every candidate;
-// the launch shape (workgroups per CU, static or dynamic queue) is then
-// timed over the whole batch.  The table can be saved to and loaded from a
-// text file keyed by the device name.
+// Autotuner.  The launch shape (workgroups per CU, static or atomic queue)
+// and the tile shape (C, seg) per width class and then per radius are chosen
+// by coordinate descent, each candidate timed on whole-batch steps.  The
+// table can be saved to and loaded from a text file keyed by the device.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -58,16 +58,42 @@ struct BItem {
     int grid, xs, y0, y1;
 };
 
+constexpr int kXcd = 8;  // workgroups are dealt round-robin to the 8 XCDs
+
+// Static mode (ctr == nullptr): logical wave w (XCD-major, xcd_tile) takes
+// items w, w+W, w+2W, ...  Dynamic mode: the items are split into kXcd
+// contiguous, cost-balanced ranges [bounds[x], bounds[x+1]); the waves of
+// XCD x (blockIdx % kXcd == x) take their first item statically and every
+// further one from that XCD's counter ctr[x], so neighbouring items stay in
+// one L2 while the load is balanced within the XCD (requires gridDim.x to
+// be a multiple of kXcd).  With ctr but no bounds: one global counter.
 template <bool kFold>
 __global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict__ grids,
                                                          const BItem *__restrict__ items, int nitems, int parity,
-                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
+                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr,
+                                                         const int *__restrict__ bounds) {
     __shared__ __attribute__((aligned(16))) float lds[4][2 * kBatchLdsw];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int nw = gridDim.x * 4;
-    int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave);
-    while (i < nitems) {
+    int i, end, stride;
+    if (ctr && !bounds) {  // one global queue
+        end = nitems;
+        stride = gridDim.x * 4;
+        i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave);
+    } else if (ctr) {
+        const int x = blockIdx.x % kXcd;
+        const int waves_x = (gridDim.x / kXcd) * 4;
+        const int s0 = bounds[x];
+        end = bounds[x + 1];
+        stride = s0 + waves_x;
+        i = __builtin_amdgcn_readfirstlane(s0 + (int)(blockIdx.x / kXcd) * 4 + wave);
+        ctr += x;
+    } else {
+        end = nitems;
+        stride = gridDim.x * 4;
+        i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave);
+    }
+    while (i < end) {
         const BItem it = items[i];
         const BGrid g = grids[it.grid];
         LaneSum ls;
@@ -81,9 +107,9 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict
         if (ctr) {
             unsigned nx = 0;
             if (lane == 0) nx = atomicAdd(ctr, 1u);
-            i = nw + __builtin_amdgcn_readfirstlane((int)nx);
+            i = stride + __builtin_amdgcn_readfirstlane((int)nx);
         } else {
-            i += nw;
+            i += stride;
         }
     }
 }
@@ -135,8 +161,9 @@ struct Batch {
     BGrid *d_grids = nullptr;
     BItem *d_items = nullptr;
     float2 *d_partials = nullptr;
-    unsigned *d_ctr = nullptr;  // one counter per step
+    unsigned *d_ctr = nullptr;  // kXcd counters per step
     int ctr_cap = 0;
+    int *d_bounds = nullptr;    // kXcd + 1 item-range bounds
     int nitems = 0, items_cap = 0;
     std::map<std::pair<int, const void *>, hipGraphExec_t> graphs;
 };
@@ -217,6 +244,25 @@ static int upload(Batch &b, const std::vector<BGrid> &hg, const std::vector<BIte
         MSL_HIP_TRY(hipMemcpyAsync(b.d_grids, hg.data(), sizeof(BGrid) * hg.size(), hipMemcpyHostToDevice,
                                    ctx().stream));
     if (ni) MSL_HIP_TRY(hipMemcpyAsync(b.d_items, hi.data(), sizeof(BItem) * ni, hipMemcpyHostToDevice, ctx().stream));
+    // cost-balanced contiguous item ranges, one per XCD: cost = cells read
+    // by the item (its rows plus the 2r halo rows, times its columns)
+    std::vector<double> cum(ni + 1, 0.0);
+    for (int i = 0; i < ni; i++) {
+        const BItem &it = hi[i];
+        const BGrid &g = hg[it.grid];
+        const int sw = g.kind == kKindNarrow1 ? 64 : 256;
+        cum[i + 1] = cum[i] + (double)(it.y1 - it.y0 + 2 * g.r) * (double)std::min(sw, g.cols - it.xs);
+    }
+    int bounds[kXcd + 1];
+    bounds[0] = 0;
+    for (int x = 1; x < kXcd; x++) {
+        const double target = cum[ni] * x / kXcd;
+        bounds[x] = (int)(std::lower_bound(cum.begin(), cum.end(), target) - cum.begin());
+        bounds[x] = std::max(bounds[x - 1], std::min(bounds[x], ni));
+    }
+    bounds[kXcd] = ni;
+    if (!b.d_bounds) MSL_HIP_TRY(hipMalloc(&b.d_bounds, sizeof(int) * (kXcd + 1)));
+    MSL_HIP_TRY(hipMemcpyAsync(b.d_bounds, bounds, sizeof(bounds), hipMemcpyHostToDevice, ctx().stream));
     MSL_HIP_TRY(hipStreamSynchronize(ctx().stream));
     b.nitems = ni;
     return MSL_OK;
@@ -229,10 +275,15 @@ static int rebuild(Batch &b) {
     return upload(b, hg, hi);
 }
 
+// Queue modes: 0 static, 1 one atomic queue per XCD, 2 one global queue.
+static const int *queue_bounds(const Batch &b) { return b.dynamic == 1 ? b.d_bounds : nullptr; }
+
 static int launch_blocks(const Batch &b) {
     const int want = (b.nitems + 3) / 4;
     const int cap = ctx().num_cus * std::max(1, std::min(b.blocks_per_cu, b.max_blocks_per_cu));
-    return std::max(1, std::min(want, cap));
+    int blocks = std::max(1, std::min(want, cap));
+    if (b.dynamic == 1) blocks = (blocks + kXcd - 1) / kXcd * kXcd;  // per-XCD queues
+    return blocks;
 }
 
 static int ensure_ctr(Batch &b, int n) {
@@ -252,17 +303,19 @@ static int enqueue_run(Batch &b, int iters, double *d_sums) {
     const int blocks = launch_blocks(b);
     hipStream_t s = ctx().stream;
     if (b.dynamic) {
-        batch_zero_kernel<<<(iters + 255) / 256, 256, 0, s>>>(b.d_ctr, iters);
+        batch_zero_kernel<<<(iters * kXcd + 255) / 256, 256, 0, s>>>(b.d_ctr, iters * kXcd);
         int rc = check_launch("msl_batch_run");
         if (rc) return rc;
     }
     for (int it = 0; it < iters; it++) {
-        unsigned *ctr = b.dynamic ? b.d_ctr + it : nullptr;
+        unsigned *ctr = b.dynamic ? b.d_ctr + it * kXcd : nullptr;
         const bool fold = d_sums && it == iters - 1;
         if (fold)
-            batch_step_kernel<true><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr);
+            batch_step_kernel<true><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr,
+                                                           queue_bounds(b));
         else
-            batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr);
+            batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr,
+                                                            queue_bounds(b));
         int rc = check_launch("msl_batch_run");
         if (rc) return rc;
     }
@@ -275,20 +328,20 @@ static int enqueue_run(Batch &b, int iters, double *d_sums) {
 
 // Mean time of `reps` back-to-back steps (parity 0, no fold), in ms.
 static int time_steps(Batch &b, int reps, float *ms) {
-    int rc = ensure_ctr(b, reps);
+    int rc = ensure_ctr(b, reps * kXcd);
     if (rc) return rc;
     const int blocks = launch_blocks(b);
     hipStream_t s = ctx().stream;
     for (int w = 0; w < 2; w++) {  // warm-up: code, TLB, caches
-        if (b.dynamic) batch_zero_kernel<<<1, 256, 0, s>>>(b.d_ctr, 1);
+        if (b.dynamic) batch_zero_kernel<<<1, 256, 0, s>>>(b.d_ctr, kXcd);
         batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, 0, b.d_partials,
-                                                        b.dynamic ? b.d_ctr : nullptr);
+                                                        b.dynamic ? b.d_ctr : nullptr, queue_bounds(b));
     }
-    if (b.dynamic) batch_zero_kernel<<<(reps + 255) / 256, 256, 0, s>>>(b.d_ctr, reps);
+    if (b.dynamic) batch_zero_kernel<<<(reps * kXcd + 255) / 256, 256, 0, s>>>(b.d_ctr, reps * kXcd);
     MSL_HIP_TRY(hipEventRecord(ctx().t0, s));
     for (int k = 0; k < reps; k++)
         batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, 0, b.d_partials,
-                                                        b.dynamic ? b.d_ctr + k : nullptr);
+                                                        b.dynamic ? b.d_ctr + k * kXcd : nullptr, queue_bounds(b));
     MSL_HIP_TRY(hipEventRecord(ctx().t1, s));
     rc = check_launch("msl_batch_autotune");
     if (rc) return rc;
@@ -332,7 +385,7 @@ static bool load_table(Batch &b, const char *path, int *entries) {
     }
     std::memcpy(b.table, t, sizeof(t));
     b.blocks_per_cu = bpc;
-    b.dynamic = dyn ? 1 : 0;
+    b.dynamic = (dyn >= 0 && dyn <= 2) ? dyn : 0;
     *entries = n;
     return true;
 }
@@ -347,67 +400,93 @@ static void save_table(const Batch &b, const char *path) {
     f << "launch " << b.blocks_per_cu << " " << b.dynamic << "\n";
 }
 
-static int autotune(Batch &b, int *entries) {
-    static const Tile kCands[] = {{4, 16}, {4, 32}, {4, 64}, {4, 128}, {1, 16}, {1, 32}, {1, 64}, {1, 128}};
-    const int reps = 5;
-    int tuned = 0;
-    std::vector<std::vector<int>> members((kBatchMaxR + 1) * kWidthClasses);
-    for (int g = 0; g < (int)b.grids.size(); g++) {
-        const msl_batch_grid &G = b.grids[g];
-        if (G.rows > 0 && G.cols > 0) members[G.r * kWidthClasses + width_class(G.cols)].push_back(g);
-    }
-    const int saved_bpc = b.blocks_per_cu, saved_dyn = b.dynamic;
-    b.dynamic = 0;
-    for (int r = 1; r <= kBatchMaxR; r++) {
-        for (int wc = 0; wc < kWidthClasses; wc++) {
-            const std::vector<int> &pick = members[r * kWidthClasses + wc];
-            if (pick.empty()) continue;
-            float best = 1e30f;
-            Tile best_t = b.table[r][wc];
-            for (const Tile &cand : kCands) {
-                Tile trial[kBatchMaxR + 1][kWidthClasses];
-                std::memcpy(trial, b.table, sizeof(trial));
-                trial[r][wc] = cand;
-                std::vector<BGrid> hg;
-                std::vector<BItem> hi;
-                plan_items(b, pick, trial, hg, hi);
-                int rc = upload(b, hg, hi);
-                if (rc) return rc;
-                float ms = 0.0f;
-                rc = time_steps(b, reps, &ms);
-                if (rc) return rc;
-                if (ms < best) {
-                    best = ms;
-                    best_t = cand;
-                }
-            }
-            b.table[r][wc] = best_t;
-            tuned++;
-        }
-    }
-    // launch shape over the whole batch
+// Whole-batch objective: the smallest of three averages of `reps`
+// back-to-back steps (the minimum is the least noisy estimate of what a
+// configuration can do).
+static int objective(Batch &b, float *ms) {
     int rc = rebuild(b);
     if (rc) return rc;
     float best = 1e30f;
-    int best_bpc = saved_bpc, best_dyn = saved_dyn;
-    for (int bpc = 1; bpc <= b.max_blocks_per_cu; bpc++) {
-        for (int dyn = 0; dyn <= 1; dyn++) {
-            b.blocks_per_cu = bpc;
-            b.dynamic = dyn;
-            float ms = 0.0f;
-            rc = time_steps(b, reps, &ms);
+    for (int k = 0; k < 3; k++) {
+        float t = 0.0f;
+        rc = time_steps(b, 3, &t);
+        if (rc) return rc;
+        best = std::min(best, t);
+    }
+    *ms = best;
+    return MSL_OK;
+}
+
+// Coordinate descent on the whole batch, so every candidate is judged with
+// the device full of the other classes' work (a class timed alone under-fills
+// the device and favours short segments):
+//   1. launch shape: workgroups per CU (the two largest that fit) x queue;
+//   2. per width class, for all radii together: C in {1, 4} x seg in
+//      {32, 64, 128};
+//   3. per radius: seg in {32, 64, 128} with the chosen C.
+// A candidate replaces the current setting only when it is at least 1 %
+// faster.
+static int autotune(Batch &b, int *entries) {
+    bool present[kBatchMaxR + 1][kWidthClasses] = {};
+    bool any_r[kBatchMaxR + 1] = {};
+    for (const msl_batch_grid &G : b.grids)
+        if (G.rows > 0 && G.cols > 0) {
+            present[G.r][width_class(G.cols)] = true;
+            any_r[G.r] = true;
+        }
+    float cur = 0.0f;
+    int rc = objective(b, &cur);
+    if (rc) return rc;
+    auto try_change = [&](auto apply, auto undo) -> int {
+        apply();
+        float t = 0.0f;
+        int trc = objective(b, &t);
+        if (trc) return trc;
+        if (t < 0.99f * cur) cur = t;
+        else undo();
+        return MSL_OK;
+    };
+    // 1. launch shape
+    for (int bpc = std::max(1, b.max_blocks_per_cu - 1); bpc <= b.max_blocks_per_cu; bpc++) {
+        for (int dyn = 0; dyn <= 2; dyn++) {
+            const int obpc = b.blocks_per_cu, odyn = b.dynamic;
+            if (bpc == obpc && dyn == odyn) continue;
+            rc = try_change([&] { b.blocks_per_cu = bpc; b.dynamic = dyn; },
+                            [&] { b.blocks_per_cu = obpc; b.dynamic = odyn; });
             if (rc) return rc;
-            if (ms < best) {
-                best = ms;
-                best_bpc = bpc;
-                best_dyn = dyn;
-            }
         }
     }
-    b.blocks_per_cu = best_bpc;
-    b.dynamic = best_dyn;
+    // 2. per width class
+    static const Tile kClassCands[] = {{4, 32}, {4, 64}, {4, 128}, {1, 32}, {1, 64}, {1, 128}};
+    for (int wc = 0; wc < kWidthClasses; wc++) {
+        bool used = false;
+        for (int r = 1; r <= kBatchMaxR; r++) used |= present[r][wc];
+        if (!used) continue;
+        for (const Tile &cand : kClassCands) {
+            Tile saved[kBatchMaxR + 1];
+            for (int r = 0; r <= kBatchMaxR; r++) saved[r] = b.table[r][wc];
+            rc = try_change([&] { for (int r = 1; r <= kBatchMaxR; r++) b.table[r][wc] = cand; },
+                            [&] { for (int r = 0; r <= kBatchMaxR; r++) b.table[r][wc] = saved[r]; });
+            if (rc) return rc;
+        }
+    }
+    // 3. per radius
+    static const int kSegs[] = {32, 64, 128};
+    for (int r = 1; r <= kBatchMaxR; r++) {
+        if (!any_r[r]) continue;
+        for (int seg : kSegs) {
+            Tile saved[kWidthClasses];
+            for (int wc = 0; wc < kWidthClasses; wc++) saved[wc] = b.table[r][wc];
+            rc = try_change([&] { for (int wc = 0; wc < kWidthClasses; wc++) b.table[r][wc].seg = seg; },
+                            [&] { for (int wc = 0; wc < kWidthClasses; wc++) b.table[r][wc] = saved[wc]; });
+            if (rc) return rc;
+        }
+    }
+    int tuned = 0;
+    for (int r = 1; r <= kBatchMaxR; r++)
+        for (int wc = 0; wc < kWidthClasses; wc++) tuned += present[r][wc] ? 1 : 0;
     *entries = tuned;
-    return MSL_OK;
+    return rebuild(b);
 }
 
 static int check_batch(void *h, const char *what) {
@@ -442,11 +521,13 @@ int msl_batch_create(const msl_batch_grid *grids, int ngrids, void **h_batch) {
     }
     Batch *b = new Batch();
     b->grids.assign(grids, grids + ngrids);
-    b->blocks_per_cu = tune_int("MSL_BATCH_BLOCKS_PER_CU", 2);
-    b->dynamic = tune_int("MSL_BATCH_DYNAMIC", 0);
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_step_kernel<true>, 256, 0) == hipSuccess && occ > 0)
         b->max_blocks_per_cu = occ;
+    b->blocks_per_cu = tune_int("MSL_BATCH_BLOCKS_PER_CU", b->max_blocks_per_cu);
+    b->dynamic = tune_int("MSL_BATCH_DYNAMIC", 1);
+    for (int r = 0; r <= kBatchMaxR; r++)
+        for (int wc = 0; wc < kWidthClasses; wc++) b->table[r][wc] = wc < 2 ? Tile{1, 64} : Tile{4, 64};
     if (ngrids > 0) {
         hipError_t e = hipMalloc(&b->d_grids, sizeof(BGrid) * ngrids);
         if (e != hipSuccess) {
@@ -473,6 +554,7 @@ int msl_batch_destroy(void *batch) {
         if (b->d_items) (void)hipFree(b->d_items);
         if (b->d_partials) (void)hipFree(b->d_partials);
         if (b->d_ctr) (void)hipFree(b->d_ctr);
+        if (b->d_bounds) (void)hipFree(b->d_bounds);
     }
     delete b;
     return MSL_OK;
@@ -497,7 +579,8 @@ int msl_batch_set_launch(void *batch, int blocks_per_cu, int dynamic_queue) {
     Batch *b = static_cast<Batch *>(batch);
     drop_graphs(*b);
     b->blocks_per_cu = blocks_per_cu;
-    b->dynamic = dynamic_queue ? 1 : 0;
+    MSL_REQUIRE(dynamic_queue >= 0 && dynamic_queue <= 2, "msl_batch_set_launch: queue mode must be 0, 1 or 2");
+    b->dynamic = dynamic_queue;
     return MSL_OK;
 }
 
@@ -548,7 +631,7 @@ int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b) {
     Batch *b = static_cast<Batch *>(batch);
     if (h_result_in_b) *h_result_in_b = iters & 1;
     if (b->grids.empty()) return MSL_OK;
-    rc = ensure_ctr(*b, std::max(iters, 1));
+    rc = ensure_ctr(*b, std::max(iters, 1) * kXcd);
     if (rc) return rc;
     const auto key = std::make_pair(iters, (const void *)d_sums);
     auto it = b->graphs.find(key);

@@ -20,16 +20,19 @@ constexpr int kTX = 64;  // x extent of a wave tile (one cell per lane)
 constexpr int kTY = 8;   // y extent of a wave tile (cells per lane)
 constexpr int kP = 4;    // plane ring: z-1, z, z+1 and the requested z+2
 
-// Branch-free cell read: the address is clamped into the cube and the value
-// replaced by nv when the cell lies outside, so loads are never predicated
-// and the compiler's memory-counter waits stay exact across the march.
-__device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, int y, int z, int nx, int ny, int nz,
-                                         float nv) {
+// Branch-free cell read: the address is clamped into the cube so the load is
+// never predicated; cell_ok tells afterwards whether the cell was inside (its
+// value is replaced by nv when it was not).  The march applies the fix-up
+// when a plane is first used, not when it is requested, so requested planes
+// stay in flight.
+__device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, int y, int z, int nx, int ny, int nz) {
     const int xc = min(max(x, 0), nx - 1);
     const int yc = min(max(y, 0), ny - 1);
     const int zc = min(max(z, 0), nz - 1);
-    const float v = in[((size_t)zc * ny + yc) * nx + xc];
-    return (x == xc && y == yc && z == zc) ? v : nv;
+    return in[((size_t)zc * ny + yc) * nx + xc];
+}
+__device__ __forceinline__ bool cell_ok(int x, int y, int z, int nx, int ny, int nz) {
+    return (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && (unsigned)z < (unsigned)nz;
 }
 
 // Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane
@@ -57,12 +60,20 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
     float hal[kTY][kP];
     auto load_plane = [&](int zz, int slot) {
 #pragma unroll
-        for (int j = 0; j < kTY + 2; j++) col[j][slot] = ld_cell(in, x, y0 - 1 + j, zz, nx, ny, nz, nv);
+        for (int j = 0; j < kTY + 2; j++) col[j][slot] = ld_cell(in, x, y0 - 1 + j, zz, nx, ny, nz);
 #pragma unroll
-        for (int j = 0; j < kTY; j++) hal[j][slot] = ld_cell(in, hxl, y0 + j, zz, nx, ny, nz, nv);
+        for (int j = 0; j < kTY; j++) hal[j][slot] = ld_cell(in, hxl, y0 + j, zz, nx, ny, nz);
+    };
+    auto fix_plane = [&](int zz, int slot) {
+#pragma unroll
+        for (int j = 0; j < kTY + 2; j++) col[j][slot] = cell_ok(x, y0 - 1 + j, zz, nx, ny, nz) ? col[j][slot] : nv;
+#pragma unroll
+        for (int j = 0; j < kTY; j++) hal[j][slot] = cell_ok(hxl, y0 + j, zz, nx, ny, nz) ? hal[j][slot] : nv;
     };
 #pragma unroll
     for (int k = 0; k < kP - 1; k++) load_plane(z0 - 1 + k, k);
+    fix_plane(z0 - 1, 0);
+    fix_plane(z0, 1);
 
     // Whole blocks of kP planes; planes past z1 are computed, not stored.
     for (int zb = z0; zb < z1; zb += kP) {
@@ -72,8 +83,10 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
             const int sb = u & (kP - 1);        // z-1
             const int sc = (u + 1) & (kP - 1);  // z
             const int sa = (u + 2) & (kP - 1);  // z+1
-            // request plane z+2 into the slot of z-2 (free since step z-1)
+            // request plane z+2 into the slot of z-2 (free since step z-1);
+            // plane z+1 is used for the first time now
             load_plane(z + 2, (u + 3) & (kP - 1));
+            fix_plane(z + 1, sa);
 #pragma unroll
             for (int j = 0; j < kTY; j++) {
                 const float c = col[j + 1][sc];

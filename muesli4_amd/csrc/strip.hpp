@@ -17,7 +17,8 @@
 //
 // Row access modes:
 //   kAligned   - cols % C == 0 and 4C-byte aligned rows: a C-vector is wholly
-//                inside or outside the grid, one load per vector;
+//                inside or outside the grid, one load per vector (the vector
+//                at c is inside exactly when 0 <= c and c + C <= cols);
 //   kUnaligned - C = 4, any cols >= 4: the load is clamped into the row (a
 //                dword-aligned 16-byte load) and each column picks its value
 //                from the loaded quad by the clamping shift, so no lane
@@ -86,16 +87,34 @@ __device__ __forceinline__ VecN<T, C> load_dword_aligned(const T *p) {
     return to_vecn<T, C>(x);
 }
 
+// Reads of C columns [c, c+C) of row yy are split into the load itself
+// (load_vec: the row and column are clamped so the address is always valid)
+// and the fix-up (fix_vec: columns outside the grid become nv, and in
+// kUnaligned mode each column picks its element of the clamped vector).
+// The march issues the load D rows ahead and applies the fix-up only when
+// the row is used, so the prefetched loads are never waited on early.
 template <typename T, int C, int MODE>
-__device__ __forceinline__ VecN<T, C> read_vec(const T *__restrict__ in, int yy, int c, int rows, int cols, T nv) {
+__device__ __forceinline__ VecN<T, C> load_vec(const T *__restrict__ in, int yy, int c, int rows, int cols) {
     const int yc = min(max(yy, 0), rows - 1);
-    const bool rowok = yy == yc;
     const T *rowp = in + (size_t)yc * cols;
+    if constexpr (MODE == kAligned) {
+        return load_aligned<T, C>(rowp + min(max(c, 0), cols - C));
+    } else if constexpr (MODE == kUnaligned) {
+        return load_dword_aligned<T, C>(rowp + min(max(c, 0), cols - C));
+    } else {
+        VecN<T, C> q;
+#pragma unroll
+        for (int k = 0; k < C; k++) q.v[k] = rowp[min(max(c + k, 0), cols - 1)];
+        return q;
+    }
+}
+
+template <typename T, int C, int MODE>
+__device__ __forceinline__ VecN<T, C> fix_vec(const VecN<T, C> &v, int yy, int c, int rows, int cols, T nv) {
+    const bool rowok = yy >= 0 && yy < rows;
     VecN<T, C> q;
     if constexpr (MODE == kAligned) {
-        const int cc = min(max(c, 0), cols - C);
-        const bool ok = rowok && cc == c;
-        const VecN<T, C> v = load_aligned<T, C>(rowp + cc);
+        const bool ok = rowok && c >= 0 && c + C <= cols;  // wholly inside or wholly outside
 #pragma unroll
         for (int k = 0; k < C; k++) q.v[k] = ok ? v.v[k] : nv;
     } else if constexpr (MODE == kUnaligned) {
@@ -103,7 +122,6 @@ __device__ __forceinline__ VecN<T, C> read_vec(const T *__restrict__ in, int yy,
         // loaded vector whenever it is inside the grid.
         const int cc = min(max(c, 0), cols - C);
         const int s = c - cc;
-        const VecN<T, C> v = load_dword_aligned<T, C>(rowp + cc);
 #pragma unroll
         for (int k = 0; k < C; k++) {
             const int j = k + s;
@@ -116,10 +134,8 @@ __device__ __forceinline__ VecN<T, C> read_vec(const T *__restrict__ in, int yy,
     } else {
 #pragma unroll
         for (int k = 0; k < C; k++) {
-            const int ck = c + k;
-            const int cc = min(max(ck, 0), cols - 1);
-            const T v = rowp[cc];
-            q.v[k] = (rowok && cc == ck) ? v : nv;
+            const int col = c + k;
+            q.v[k] = (rowok && col >= 0 && col < cols) ? v.v[k] : nv;
         }
     }
     return q;
@@ -230,7 +246,7 @@ struct LaneSum {
 
 // March one wave down its (strip, segment): input rows y0-R .. yend-1+R,
 // output rows y0 .. yend-1 of the strip starting at column xs.  One code
-// path for every tile: reads are clamped (read_vec), stores are masked per
+// path for every tile: reads are clamped (load_vec / fix_vec), stores are masked per
 // lane, and the loop is unrolled over exactly one ring period so the code
 // stays small (the instruction cache is shared by two CUs) while every
 // register index is a compile-time constant.  Row t+D is requested when row t
@@ -263,11 +279,11 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
 
     VecN<T, C> ring[PER];  // row sums
     VecN<T, C> cring[Pol::kCenter ? PER : 1];
-    VecN<T, C> qc[D], qh[D];  // requested rows t .. t+D-1 (centre and halo vectors)
+    VecN<T, C> qc[D], qh[D];  // raw loads of rows t .. t+D-1 (centre and halo vectors)
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        qc[d] = read_vec<T, C, MODE>(in, y0 - R + d, x0, rows, cols, nv);
-        qh[d] = read_vec<T, C, MODE>(in, y0 - R + d, hx, rows, cols, nv);
+        qc[d] = load_vec<T, C, MODE>(in, y0 - R + d, x0, rows, cols);
+        qh[d] = load_vec<T, C, MODE>(in, y0 - R + d, hx, rows, cols);
     }
 
     for (int tb = 0; tb < total; tb += PER) {
@@ -277,11 +293,13 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
             const int qs = u % D;  // queue slot of row t
             T *buf = lds[t & 1];
             typedef typename NativeVecN<T, C>::type V;
-            *reinterpret_cast<V *>(buf + RP + C * lane) = to_native<T, C>(qc[qs]);
-            *reinterpret_cast<V *>(buf + hslot) = to_native<T, C>(qh[qs]);
+            const int yy = y0 - R + t;
+            *reinterpret_cast<V *>(buf + RP + C * lane) =
+                to_native<T, C>(fix_vec<T, C, MODE>(qc[qs], yy, x0, rows, cols, nv));
+            *reinterpret_cast<V *>(buf + hslot) = to_native<T, C>(fix_vec<T, C, MODE>(qh[qs], yy, hx, rows, cols, nv));
             // request row t+D into the slot just staged
-            qc[qs] = read_vec<T, C, MODE>(in, y0 - R + t + D, x0, rows, cols, nv);
-            qh[qs] = read_vec<T, C, MODE>(in, y0 - R + t + D, hx, rows, cols, nv);
+            qc[qs] = load_vec<T, C, MODE>(in, yy + D, x0, rows, cols);
+            qh[qs] = load_vec<T, C, MODE>(in, yy + D, hx, rows, cols);
             wave_lds_sync();
             T w[C * NQ];
 #pragma unroll

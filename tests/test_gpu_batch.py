@@ -89,9 +89,9 @@ def test_batch_loop_baseline_matches_oracle():
 
 
 @pytest.mark.parametrize("cpl,seg", [(4, 16), (4, 128), (1, 16), (1, 64), (4, 7), (1, 1000)])
-@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("dynamic", [0, 1, 2])
 def test_batch_tiles_and_queue_modes(cpl, seg, dynamic):
-    note(f"batch tiles cpl={cpl} seg={seg} dynamic={dynamic}")
+    note(f"batch tiles cpl={cpl} seg={seg} queue={dynamic}")
     b = small_batch(seed=7)
     for r in range(1, 11):
         for wc in range(4):
@@ -125,7 +125,7 @@ def test_batch_graph_replay_is_stable():
     bits every time (the queue counters are re-zeroed inside the graph)."""
     note("batch graph replay")
     b = small_batch(seed=9)
-    b.set_launch(2, True)
+    b.set_launch(2, 1)
     outs = []
     for _ in range(3):
         b.reset()
