@@ -55,7 +55,8 @@ struct BGrid {
 };
 
 struct BItem {
-    int grid, xs, y0, y1;
+    int grid2;  // 2 * grid + march direction (1 = upwards)
+    int xs, y0, y1;
 };
 
 constexpr int kXcd = 8;  // workgroups are dealt round-robin to the 8 XCDs
@@ -95,11 +96,12 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict
     }
     while (i < end) {
         const BItem it = items[i];
-        const BGrid g = grids[it.grid];
+        const BGrid g = grids[it.grid2 >> 1];
         LaneSum ls;
         const float *src = parity ? g.b : g.a;
         float *dst = parity ? g.a : g.b;
-        blur_item_r<kFold>(g.r, src, dst, g.rows, g.cols, g.nv, g.kind, it.xs, it.y0, it.y1, lds[wave], &ls);
+        blur_item_r<kFold>(g.r, src, dst, g.rows, g.cols, g.nv, g.kind, it.xs, it.y0, it.y1, lds[wave], &ls,
+                           it.grid2 & 1);
         if constexpr (kFold) {
             wave_pair_reduce(ls);
             if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
@@ -221,7 +223,8 @@ static void plan_items(const Batch &b, const std::vector<int> &pick, const Tile 
         d.item0 = (int)hi.size();
         if (G.rows > 0 && G.cols > 0) {
             for (int xs = 0; xs < G.cols; xs += sw)
-                for (int y0 = 0; y0 < G.rows; y0 += seg) hi.push_back(BItem{g, xs, y0, std::min(y0 + seg, G.rows)});
+                for (int y0 = 0; y0 < G.rows; y0 += seg)
+                    hi.push_back(BItem{2 * g + ((y0 / seg) & 1), xs, y0, std::min(y0 + seg, G.rows)});
         }
         d.nitems = (int)hi.size() - d.item0;
     }
@@ -249,7 +252,7 @@ static int upload(Batch &b, const std::vector<BGrid> &hg, const std::vector<BIte
     std::vector<double> cum(ni + 1, 0.0);
     for (int i = 0; i < ni; i++) {
         const BItem &it = hi[i];
-        const BGrid &g = hg[it.grid];
+        const BGrid &g = hg[it.grid2 >> 1];
         const int sw = g.kind == kKindNarrow1 ? 64 : 256;
         cum[i + 1] = cum[i] + (double)(it.y1 - it.y0 + 2 * g.r) * (double)std::min(sw, g.cols - it.xs);
     }

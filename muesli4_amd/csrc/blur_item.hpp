@@ -33,41 +33,42 @@ inline int width_class(int cols) {
 
 // One work item: rows [y0, y1) of the strip starting at column xs of a
 // rows x cols grid, read from src and written to dst, with the row mode of
-// `kind`; radii above kBatchMaxR4 always run with one column per lane.
+// `kind`, marching upwards when `up`; radii above kBatchMaxR4 always run
+// with one column per lane.
 template <int R, bool kFold>
 __device__ __forceinline__ void blur_item(const float *src, float *dst, int rows, int cols, float nv, int kind, int xs,
-                                          int y0, int y1, float *ldsw, LaneSum *ls) {
+                                          int y0, int y1, float *ldsw, LaneSum *ls, int up) {
     const MeanPolicy<R> pol = MeanPolicy<R>::make();
     if constexpr (R > kBatchMaxR4) {
         strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
-            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls);
+            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls, up);
     } else if (kind == kKindAligned4) {
         strip_march<MeanPolicy<R>, 4, kAligned, kFold, BatchDepth<R, 4>::D>(
-            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls);
+            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls, up);
     } else if (kind == kKindUnaligned4) {
         strip_march<MeanPolicy<R>, 4, kUnaligned, kFold, BatchDepth<R, 4>::D>(
-            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls);
+            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls, up);
     } else {
         strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
-            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls);
+            dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls, up);
     }
 }
 
 // Radius switch (r is wave-uniform, so the branch is a scalar jump).
 template <bool kFold>
 __device__ __forceinline__ void blur_item_r(int r, const float *src, float *dst, int rows, int cols, float nv,
-                                            int kind, int xs, int y0, int y1, float *ldsw, LaneSum *ls) {
+                                            int kind, int xs, int y0, int y1, float *ldsw, LaneSum *ls, int up) {
     switch (r) {
-        case 1: blur_item<1, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 2: blur_item<2, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 3: blur_item<3, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 4: blur_item<4, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 5: blur_item<5, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 6: blur_item<6, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 7: blur_item<7, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 8: blur_item<8, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        case 9: blur_item<9, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
-        default: blur_item<10, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls); break;
+        case 1: blur_item<1, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 2: blur_item<2, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 3: blur_item<3, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 4: blur_item<4, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 5: blur_item<5, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 6: blur_item<6, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 7: blur_item<7, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 8: blur_item<8, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        case 9: blur_item<9, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
+        default: blur_item<10, kFold>(src, dst, rows, cols, nv, kind, xs, y0, y1, ldsw, ls, up); break;
     }
 }
 

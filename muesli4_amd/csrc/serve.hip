@@ -105,15 +105,16 @@ __global__ __launch_bounds__(256) void serve_blur_kernel(const SBlur *__restrict
         const int k = i - g.item0;
         const int sw = g.kind == kKindNarrow1 ? 64 : 256;
         const int xs = (k % g.strips_x) * sw;
-        const int y0 = (k / g.strips_x) * g.seg;
+        const int sy = k / g.strips_x;
+        const int y0 = sy * g.seg;
         const int y1 = min(y0 + g.seg, g.rows);
         LaneSum ls;
         if (g.fold >= 0) {
-            blur_item_r<true>(g.r, g.src, g.dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls);
+            blur_item_r<true>(g.r, g.src, g.dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls, sy & 1);
             wave_pair_reduce(ls);
             if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
         } else {
-            blur_item_r<false>(g.r, g.src, g.dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls);
+            blur_item_r<false>(g.r, g.src, g.dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls, sy & 1);
         }
     }
 }

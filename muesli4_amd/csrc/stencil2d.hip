@@ -39,7 +39,8 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     const int sx = item % strips_x;
     const int sy = item / strips_x;
     const int y0 = sy * seg;
-    strip_march<Pol, 4, MODE>(out, in, rows, cols, nv, sx * kStripW, y0, min(y0 + seg, rows), pol, lds[wave]);
+    strip_march<Pol, 4, MODE>(out, in, rows, cols, nv, sx * kStripW, y0, min(y0 + seg, rows), pol, lds[wave], nullptr,
+                              sy & 1);
 }
 
 // Row access mode of a 4-column-per-lane march over these buffers.

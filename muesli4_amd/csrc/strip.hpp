@@ -214,6 +214,8 @@ struct LifePolicy {
     __device__ __forceinline__ int32_t acc(int32_t a, int32_t b) const { return a + b; }
 };
 
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
 template <int R, int C>
 struct StripConsts {
     static_assert(C == 1 || C == 4, "1 or 4 columns per lane");
@@ -224,12 +226,12 @@ struct StripConsts {
     static constexpr int NQ = 2 * HL + 1;           // window vectors per lane
     static constexpr int PER = 2 * R + 1;           // ring period (= loop unroll)
     static_assert(2 * HL <= 64, "halo wider than a wavefront");
-    // Rows requested ahead of use: a divisor of PER (so queue slots stay
-    // static), larger where the per-row work is too short to hide latency.
-    // With one column per lane the rows are narrow and the whole period is
-    // kept in flight.
-    static constexpr int D = C == 1 ? PER
-                                    : (PER <= 7 ? PER : (R > 10 ? 1 : (PER % 5 == 0 ? 5 : (PER % 3 == 0 ? 3 : 1))));
+    // Rows requested ahead of use: larger where the per-row work is too
+    // short to hide latency (the march unrolls over lcm(PER, D) rows).  With
+    // one column per lane the rows are narrow and the whole period is kept
+    // in flight.
+    static constexpr int D =
+        C == 1 ? PER : (R == 1 ? 6 : (PER <= 7 ? PER : (R > 10 ? 1 : (PER % 5 == 0 ? 5 : (PER % 3 == 0 ? 3 : 1)))));
 };
 
 // Sum of the values one wave stores, kept as a compensated (s, c) pair per
@@ -250,13 +252,13 @@ struct LaneSum {
 // lane, and the loop is unrolled over exactly one ring period so the code
 // stays small (the instruction cache is shared by two CUs) while every
 // register index is a compile-time constant.  Row t+D is requested when row t
-// is staged (DQ > 0 overrides the default prefetch depth; it must divide
-// 2R+1).  With kSum, every stored value is also pushed into *sum.
-template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0>
-__device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
-                                            int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
-                                            const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW],
-                                            LaneSum *sum = nullptr) {
+// is staged (DQ > 0 overrides the default prefetch depth).  With kSum, every
+// stored value is also pushed into *sum.
+template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP>
+__device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ out,
+                                                const typename Pol::T *__restrict__ in, int rows, int cols,
+                                                typename Pol::T nv, int xs, int y0, int yend, const Pol &pol,
+                                                typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW], LaneSum *sum) {
     typedef typename Pol::T T;
     typedef StripConsts<Pol::R, C> K;
     constexpr int R = Pol::R;
@@ -265,11 +267,16 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
     constexpr int PER = K::PER;
     constexpr int D = DQ > 0 ? DQ : K::D;
     constexpr int HL = K::HL;
-    static_assert(PER % D == 0, "prefetch depth must divide the ring period");
+    // the loop body is unrolled over UL = lcm(PER, D) rows so that both the
+    // ring slot (t mod PER) and the queue slot (t mod D) are static
+    constexpr int UL = PER * D / gcd_c(PER, D);
 
     const int lane = threadIdx.x & 63;
     const int x0 = xs + C * lane;
     const int total = (yend - y0) + 2 * R;  // input rows y0-R .. yend-1+R
+    // input row of march step t: ascending from y0-R, or descending from yend-1+R
+    const int ybase = UP ? yend - 1 + R : y0 - R;
+    constexpr int ystep = UP ? -1 : 1;
     // Halo vectors: left ones for lanes [0, HL), right ones for lanes
     // [HL, 2HL); every other lane re-reads and re-stages its own centre
     // vector, so no lane diverges.
@@ -282,24 +289,25 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
     VecN<T, C> qc[D], qh[D];  // raw loads of rows t .. t+D-1 (centre and halo vectors)
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        qc[d] = load_vec<T, C, MODE>(in, y0 - R + d, x0, rows, cols);
-        qh[d] = load_vec<T, C, MODE>(in, y0 - R + d, hx, rows, cols);
+        qc[d] = load_vec<T, C, MODE>(in, ybase + ystep * d, x0, rows, cols);
+        qh[d] = load_vec<T, C, MODE>(in, ybase + ystep * d, hx, rows, cols);
     }
 
-    for (int tb = 0; tb < total; tb += PER) {
+    for (int tb = 0; tb < total; tb += UL) {
 #pragma unroll
-        for (int u = 0; u < PER; u++) {
-            const int t = tb + u;  // ring slot of row t is u (tb is a multiple of PER)
-            const int qs = u % D;  // queue slot of row t
+        for (int uu = 0; uu < UL; uu++) {
+            const int t = tb + uu;
+            const int u = uu % PER;  // ring slot of row t (tb is a multiple of PER)
+            const int qs = uu % D;   // queue slot of row t (tb is a multiple of D)
             T *buf = lds[t & 1];
             typedef typename NativeVecN<T, C>::type V;
-            const int yy = y0 - R + t;
+            const int yy = ybase + ystep * t;
             *reinterpret_cast<V *>(buf + RP + C * lane) =
                 to_native<T, C>(fix_vec<T, C, MODE>(qc[qs], yy, x0, rows, cols, nv));
             *reinterpret_cast<V *>(buf + hslot) = to_native<T, C>(fix_vec<T, C, MODE>(qh[qs], yy, hx, rows, cols, nv));
             // request row t+D into the slot just staged
-            qc[qs] = load_vec<T, C, MODE>(in, yy + D, x0, rows, cols);
-            qh[qs] = load_vec<T, C, MODE>(in, yy + D, hx, rows, cols);
+            qc[qs] = load_vec<T, C, MODE>(in, yy + ystep * D, x0, rows, cols);
+            qh[qs] = load_vec<T, C, MODE>(in, yy + ystep * D, hx, rows, cols);
             wave_lds_sync();
             T w[C * NQ];
 #pragma unroll
@@ -314,16 +322,19 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
 #pragma unroll
                 for (int c = 0; c < C; c++) cring[u].v[c] = w[RP + c];
             }
-            const int y = y0 - 2 * R + t;  // output row completed by row t
-            if (t >= 2 * R && y < yend) {
+            const int y = yy - ystep * R;  // output row completed by row t
+            if (t >= 2 * R && (UP ? y >= y0 : y < yend)) {
                 VecN<T, C> o;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
+                    // rows y-R .. y+R in ascending order: the oldest ring entry
+                    // first when marching down, the newest first when marching up
                     T acc = pol.colsum_init();
 #pragma unroll
-                    for (int k = 0; k < PER; k++) acc = pol.acc(acc, ring[(u + 1 + k) % PER].v[c]);
+                    for (int k = 0; k < PER; k++)
+                        acc = pol.acc(acc, ring[UP ? (u + PER - k) % PER : (u + 1 + k) % PER].v[c]);
                     T centre = T(0);
-                    if constexpr (Pol::kCenter) centre = cring[(u + 1 + R) % PER].v[c];
+                    if constexpr (Pol::kCenter) centre = cring[(u + PER - R) % PER].v[c];
                     o.v[c] = pol.finish(acc, centre);
                 }
                 write_vec<T, C, MODE>(out + (size_t)y * cols, x0, cols, o);
@@ -335,6 +346,23 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
             }
         }
     }
+}
+
+
+// March in the given direction (up != 0: from the last row upwards).  The
+// result is the same either way; alternating the direction of vertically
+// adjacent segments makes both readers of the 2R halo rows between them
+// read those rows at the same time (both at the start or both at the end
+// of their marches), so the second read is served by L2.
+template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0>
+__device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
+                                            int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
+                                            const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW],
+                                            LaneSum *sum = nullptr, int up = 0) {
+    if (up)
+        strip_march_dir<Pol, C, MODE, kSum, DQ, true>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
+    else
+        strip_march_dir<Pol, C, MODE, kSum, DQ, false>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
 }
 
 }  // namespace mslhip
