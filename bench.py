@@ -192,6 +192,7 @@ def main():
         result["heat3d"] = bench_heat(hip, call, DeviceArray)
         result["batched"] = bench_batch(hip)
         result["serving"] = bench_serve(hip)
+        result["cpp_api"] = bench_cpp_api()
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(value)
@@ -368,6 +369,22 @@ def bench_batch(hip):
            "tiles_cols_per_lane_seg_by_width_class": tiles}
     log(f"[bench] batched {out}")
     bt.close()
+    return out
+
+
+def bench_cpp_api():
+    """User functors through the C++ skeleton API (the reference examples'
+    heat3D, GoL and mean-blur functors, map/zip/fold functors) beside the
+    library kernels, measured by the compiled skeleton_bench program."""
+    import subprocess
+    exe = os.path.join(ROOT, "muesli4_amd", "bench_cpp", "build", "skeleton_bench")
+    if not os.path.exists(exe):
+        return {"error": f"{exe} not built"}
+    res = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    if res.returncode != 0:
+        return {"error": f"skeleton_bench exited with {res.returncode}: {res.stderr[-500:]}"}
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    log(f"[bench] cpp_api {out}")
     return out
 
 

@@ -116,10 +116,19 @@ def build_cpp_tests() -> str:
     return build_program(os.path.join(CPP_TESTS, "skeletons_test.cpp"), out, rpath_origin=rel)
 
 
+def build_cpp_bench() -> str:
+    """The C++ skeleton-API benchmark run by bench.py (user functors through
+    msl::DA/DM/DC beside the library kernels)."""
+    out = os.path.join(PKG, "bench_cpp", "build", "skeleton_bench")
+    rel = os.path.relpath(LIBDIR, os.path.dirname(out))
+    return build_program(os.path.join(PKG, "bench_cpp", "skeleton_bench.cpp"), out, rpath_origin=rel)
+
+
 def build_all() -> None:
     build_oracle()
     build_hip()
     build_cpp_tests()
+    build_cpp_bench()
 
 
 if __name__ == "__main__":

@@ -79,13 +79,16 @@ public:
         this->updateDevice();
         if (this->n == 0) return;
         const int r = (int)stencilSize;
-        const int tx = (ncol + detail::kTile3X - 1) / detail::kTile3X;
-        const int ty = (nrow + detail::kTile3Y - 1) / detail::kTile3Y;
-        const int tz = (depth + detail::kTile3Z - 1) / detail::kTile3Z;
-        const size_t lds = detail::stencil3d_lds_bytes(r, sizeof(T));
-        const int use_lds = lds <= detail::kMaxLdsBytes;
-        detail::dc_stencil_kernel<T, f><<<tx * ty * tz, detail::kThreads, use_lds ? lds : 0, detail::stream()>>>(
-            result.d_data, this->d_data, ncol, nrow, depth, r, neutralValue, tx, ty, use_lds);
+        const size_t lds = detail::march3d_lds_bytes(r, sizeof(T));
+        if (lds <= detail::kMaxLdsBytes) {
+            const detail::March3D m = detail::march3d_shape(ncol, nrow, depth);
+            const int mask = detail::march3d_ring_planes(r) - 1;
+            detail::dc_stencil_kernel<T, f><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+                result.d_data, this->d_data, ncol, nrow, depth, r, neutralValue, m.xtiles, m.ytiles, m.zseg, mask);
+        } else {
+            detail::dc_stencil_global_kernel<T, f><<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                result.d_data, this->d_data, ncol, nrow, depth, r, neutralValue);
+        }
         detail::check_launch("DC::mapStencil");
         result.deviceWritten();
     }

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "../muesli.h"
 #include "../nplmatrix.h"
@@ -46,16 +47,61 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(T *out, size_t n, T v) {
     MSL_GRID_STRIDE(i, n) out[i] = v;
 }
 
-// out[i] = f(in[i])
+// Four consecutive elements of a 4-byte type, moved with one 16-byte access.
+template <typename T>
+struct alignas(16) Quad4 {
+    T v[4];
+};
+template <typename T>
+constexpr bool quad_type() {
+    return sizeof(T) == 4 && std::is_trivially_copyable<T>::value;
+}
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+#define MSL_GRID_STRIDE_FROM(i, start, n)                                                          \
+    for (size_t i = (start) + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+         i += (size_t)gridDim.x * blockDim.x)
+
+// out[i] = f(in[i]); with quad (4-byte types, 16-byte aligned buffers) four
+// elements per 16-byte load and store, the remainder element-wise.
 template <typename T, typename R, typename F>
-__global__ __launch_bounds__(kThreads) void map_kernel(R *out, const T *in, size_t n, F f) {
-    MSL_GRID_STRIDE(i, n) out[i] = f.F::operator()(in[i]);
+__global__ __launch_bounds__(kThreads) void map_kernel(R *out, const T *in, size_t n, F f, int quad) {
+    size_t start = 0;
+    if constexpr (quad_type<T>() && quad_type<R>()) {
+        if (quad) {
+            const size_t n4 = n / 4;
+            MSL_GRID_STRIDE(i, n4) {
+                const Quad4<T> a = reinterpret_cast<const Quad4<T> *>(in)[i];
+                Quad4<R> o;
+#pragma unroll
+                for (int k = 0; k < 4; k++) o.v[k] = f.F::operator()(a.v[k]);
+                reinterpret_cast<Quad4<R> *>(out)[i] = o;
+            }
+            start = n4 * 4;
+        }
+    }
+    MSL_GRID_STRIDE_FROM(i, start, n) out[i] = f.F::operator()(in[i]);
 }
 
 // out[i] = f(a[i], b[i])
 template <typename A, typename B, typename R, typename F>
-__global__ __launch_bounds__(kThreads) void zip_kernel(R *out, const A *a, const B *b, size_t n, F f) {
-    MSL_GRID_STRIDE(i, n) out[i] = f.F::operator()(a[i], b[i]);
+__global__ __launch_bounds__(kThreads) void zip_kernel(R *out, const A *a, const B *b, size_t n, F f, int quad) {
+    size_t start = 0;
+    if constexpr (quad_type<A>() && quad_type<B>() && quad_type<R>()) {
+        if (quad) {
+            const size_t n4 = n / 4;
+            MSL_GRID_STRIDE(i, n4) {
+                const Quad4<A> x = reinterpret_cast<const Quad4<A> *>(a)[i];
+                const Quad4<B> y = reinterpret_cast<const Quad4<B> *>(b)[i];
+                Quad4<R> o;
+#pragma unroll
+                for (int k = 0; k < 4; k++) o.v[k] = f.F::operator()(x.v[k], y.v[k]);
+                reinterpret_cast<Quad4<R> *>(out)[i] = o;
+            }
+            start = n4 * 4;
+        }
+    }
+    MSL_GRID_STRIDE_FROM(i, start, n) out[i] = f.F::operator()(a[i], b[i]);
 }
 
 // out[i] = f(a[i], b[i], c[i])
@@ -148,10 +194,26 @@ __device__ __forceinline__ void fold_block(T acc, bool has, F &f, T *out, int *o
 }
 
 template <typename T, typename F>
-__global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size_t n, T *partials, int *has, F f) {
+__global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size_t n, T *partials, int *has, F f,
+                                                               int quad) {
     T acc{};
     bool h = false;
-    MSL_GRID_STRIDE(i, n) {
+    size_t start = 0;
+    if constexpr (quad_type<T>()) {
+        if (quad) {  // this thread's quads in grid-stride order, then its tail elements
+            const size_t n4 = n / 4;
+            MSL_GRID_STRIDE(i, n4) {
+                const Quad4<T> a = reinterpret_cast<const Quad4<T> *>(in)[i];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    acc = h ? f.F::operator()(acc, a.v[k]) : a.v[k];
+                    h = true;
+                }
+            }
+            start = n4 * 4;
+        }
+    }
+    MSL_GRID_STRIDE_FROM(i, start, n) {
         acc = h ? f.F::operator()(acc, in[i]) : in[i];
         h = true;
     }
@@ -173,65 +235,151 @@ __global__ __launch_bounds__(kThreads) void fold_stage2_kernel(const T *partials
 }
 
 // ---- stencils ---------------------------------------------------------------------
-// 2-D: a 256-thread block computes a 64 x 16 output tile (4 rows per thread)
-// after staging the tile plus a halo of the stencil radius in LDS.
-constexpr int kTileX = 64;
-constexpr int kTileY = 16;
-
-inline size_t stencil2d_lds_bytes(int r, size_t elem) { return (size_t)(kTileX + 2 * r) * (kTileY + 2 * r) * elem; }
+// 2-D march: a 256-thread block owns a strip of kMarchW columns (one output
+// column per thread) and a segment of rows.  At each step it stages kMarchK
+// input rows of the window (the strip plus r halo columns each side, with
+// the outside value for cells off the grid) into an LDS ring of
+// H >= 2r + 2*kMarchK rows, the next step's rows being requested in
+// registers meanwhile; after one barrier every thread computes the kMarchK
+// output rows whose 2r+1 window rows are now staged.  The ring is deep
+// enough that the rows staged at the next step never overwrite a row that
+// the current step reads, so one barrier per step suffices.
+constexpr int kMarchW = 256;
+constexpr int kMarchK = 4;
 constexpr size_t kMaxLdsBytes = 48 * 1024;
 
-template <typename T>
-__device__ __forceinline__ void stage_tile_2d(T *tile, const T *in, int rows, int cols, int gx0, int gy0, int tw,
-                                              int th, T nv) {
-    for (int idx = threadIdx.x; idx < tw * th; idx += kThreads) {
-        const int lx = idx % tw, ly = idx / tw;
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        tile[idx] = ((unsigned)gx < (unsigned)cols && (unsigned)gy < (unsigned)rows) ? in[(size_t)gy * cols + gx] : nv;
+inline int pow2_at_least(int v) {
+    int h = 1;
+    while (h < v) h <<= 1;
+    return h;
+}
+
+inline int march2d_ring_rows(int r) { return pow2_at_least(2 * r + 2 * kMarchK); }
+
+inline size_t march2d_lds_bytes(int r, size_t elem) {
+    return (size_t)march2d_ring_rows(r) * (size_t)(kMarchW + 2 * r) * elem;
+}
+
+struct March2D {
+    int strips_x, seg, blocks;
+};
+
+inline March2D march2d_shape(int rows, int cols) {
+    March2D m;
+    m.strips_x = (cols + kMarchW - 1) / kMarchW;
+    const long target = (long)num_cus() * 4;
+    long segs = (target + m.strips_x - 1) / m.strips_x;
+    if (segs < 1) segs = 1;
+    int seg = (int)((rows + segs - 1) / segs);
+    if (seg < kMarchK) seg = kMarchK;
+    m.seg = seg;
+    m.blocks = m.strips_x * ((rows + seg - 1) / seg);
+    return m;
+}
+
+// Runs the march of output rows [y0, y1) of the strip at column xs and calls
+// out_row(yo) once the window rows yo-r .. yo+r are staged.  outside(row, col)
+// gives the staged value of cells off the grid.
+template <typename T, typename Outside, typename OutRow>
+__device__ __forceinline__ void march2d(const T *__restrict__ in, int rows, int cols, int r, int xs, int y0, int y1,
+                                        T *ring, int mask, Outside outside, OutRow out_row) {
+    const int t = threadIdx.x;
+    const int W = kMarchW + 2 * r;
+    const int xa = xs - r + t;
+    const bool hasb = t < 2 * r;  // this thread also stages column xa + kMarchW
+    const int xb = hasb ? xa + kMarchW : xa;
+    auto load = [&](int yy, int x) -> T {
+        const int yc = min(max(yy, 0), rows - 1), xc = min(max(x, 0), cols - 1);
+        return in[(size_t)yc * cols + xc];
+    };
+    auto fix = [&](const T &v, int yy, int x) -> T {
+        return ((unsigned)yy < (unsigned)rows && (unsigned)x < (unsigned)cols) ? v : outside(yy, x);
+    };
+    const int total = (y1 - y0) + 2 * r;  // rows to stage: y0-r .. y1-1+r
+    T qa[kMarchK], qb[kMarchK];
+#pragma unroll
+    for (int k = 0; k < kMarchK; k++) {
+        qa[k] = load(y0 - r + k, xa);
+        qb[k] = load(y0 - r + k, xb);
+    }
+    for (int sk = 0; sk < total; sk += kMarchK) {  // uniform across the block
+        const int a = y0 - r + sk;                  // first row staged at this step
+#pragma unroll
+        for (int k = 0; k < kMarchK; k++) {
+            const int yy = a + k;
+            T *rowp = ring + (yy & mask) * W;
+            rowp[t] = fix(qa[k], yy, xa);
+            if (hasb) rowp[t + kMarchW] = fix(qb[k], yy, xb);
+            qa[k] = load(yy + kMarchK, xa);
+            qb[k] = load(yy + kMarchK, xb);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kMarchK; k++) {
+            const int yo = a - r + k;  // its window rows a-2r+k .. a+k are staged
+            if (yo >= y0 && yo < y1) out_row(yo);
+        }
     }
 }
 
 template <typename T, T (*f)(const NPLMatrix<T> &, int, int)>
 __global__ __launch_bounds__(kThreads) void dm_stencil_kernel(T *out, const T *in, int rows, int cols, int r, T nv,
-                                                              int tiles_x, int use_lds) {
+                                                              int strips_x, int seg, int mask) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    T *tile = reinterpret_cast<T *>(smem_raw);
-    const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
-    const int x0 = bx * kTileX, y0 = by * kTileY;
+    T *ring = reinterpret_cast<T *>(smem_raw);
+    const int xs = (blockIdx.x % strips_x) * kMarchW;
+    const int y0 = (blockIdx.x / strips_x) * seg;
+    const int y1 = min(y0 + seg, rows);
     NPLMatrix<T> view;
     view.width = cols;
     view.height = rows;
     view.stencilSize = r;
     view.neutralValue = nv;
     view.data = in;
-    if (use_lds) {
-        const int tw = kTileX + 2 * r, th = kTileY + 2 * r;
-        stage_tile_2d(tile, in, rows, cols, x0 - r, y0 - r, tw, th, nv);
-        __syncthreads();
-        view.tile = tile;
-        view.tileX = x0 - r;
-        view.tileY = y0 - r;
-        view.tileW = tw;
-        view.tileH = th;
-    }
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int x = x0 + tx;
-#pragma unroll
-    for (int k = 0; k < kTileY / 4; k++) {
-        const int y = y0 + ty + 4 * k;
-        if (x < cols && y < rows) out[(size_t)y * cols + x] = f(view, x, y);
+    view.tile = ring;
+    view.tileX = xs - r;
+    view.tileW = kMarchW + 2 * r;
+    view.tileH = 2 * r + 1;
+    view.rowMask = mask;
+    const int x = xs + threadIdx.x;
+    // The callbacks capture the view by value (a per-row copy): capturing it
+    // by reference miscompiles for gfx950 once a large user function is
+    // inlined ("Illegal instruction detected").
+    march2d(
+        in, rows, cols, r, xs, y0, y1, ring, mask, [=](int, int) { return nv; },
+        [=](int yo) {
+            NPLMatrix<T> v = view;
+            v.tileY = yo - r;
+            if (x < cols) out[(size_t)yo * cols + x] = f(v, x, yo);
+        });
+}
+
+// Without LDS (window too large): every output reads through the view.
+template <typename T, T (*f)(const NPLMatrix<T> &, int, int)>
+__global__ __launch_bounds__(kThreads) void dm_stencil_global_kernel(T *out, const T *in, int rows, int cols, int r,
+                                                                     T nv) {
+    NPLMatrix<T> view;
+    view.width = cols;
+    view.height = rows;
+    view.stencilSize = r;
+    view.neutralValue = nv;
+    view.data = in;
+    MSL_GRID_STRIDE(k, (size_t)rows * cols) {
+        const int y = (int)(k / (size_t)cols), x = (int)(k % (size_t)cols);
+        out[k] = f(view, x, y);
     }
 }
 
 // Legacy mapStencilMM: result(row, col) = f(row, col, &view, ncol, nrow).
 template <typename T, typename R, typename F>
 __global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T *in, int rows, int cols, int r, T nv,
-                                                                 const T *border_data, int border, int tiles_x,
-                                                                 int use_lds, F f) {
+                                                                 const T *border_data, int border, int strips_x,
+                                                                 int seg, int mask, F f) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    T *tile = reinterpret_cast<T *>(smem_raw);
-    const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
-    const int x0 = bx * kTileX, y0 = by * kTileY;
+    T *ring = reinterpret_cast<T *>(smem_raw);
+    const int xs = (blockIdx.x % strips_x) * kMarchW;
+    const int y0 = (blockIdx.x / strips_x) * seg;
+    const int y1 = min(y0 + seg, rows);
     PLMatrix<T> view;
     view.rows = rows;
     view.cols = cols;
@@ -240,22 +388,35 @@ __global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T
     view.data = in;
     view.border_data = border_data;
     view.border = border;
-    if (use_lds) {
-        const int tw = kTileX + 2 * r, th = kTileY + 2 * r;
-        stage_tile_2d(tile, in, rows, cols, x0 - r, y0 - r, tw, th, nv);
-        __syncthreads();
-        view.tile = tile;
-        view.tileRow = y0 - r;
-        view.tileCol = x0 - r;
-        view.tileRows = th;
-        view.tileCols = tw;
-    }
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int col = x0 + tx;
-#pragma unroll
-    for (int k = 0; k < kTileY / 4; k++) {
-        const int row = y0 + ty + 4 * k;
-        if (col < cols && row < rows) out[(size_t)row * cols + col] = f.F::operator()(row, col, &view, cols, rows);
+    view.tile = ring;
+    view.tileCol = xs - r;
+    view.tileCols = kMarchW + 2 * r;
+    view.tileRows = 2 * r + 1;
+    view.rowMask = mask;
+    const int col = xs + threadIdx.x;
+    march2d(
+        in, rows, cols, r, xs, y0, y1, ring, mask, [=](int row, int c) { return view.outside(row, c); },
+        [=, &f](int row) {
+            PLMatrix<T> v = view;
+            v.tileRow = row - r;
+            if (col < cols) out[(size_t)row * cols + col] = f.F::operator()(row, col, &v, cols, rows);
+        });
+}
+
+template <typename T, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void dm_stencil_mm_global_kernel(R *out, const T *in, int rows, int cols, int r,
+                                                                        T nv, const T *border_data, int border, F f) {
+    PLMatrix<T> view;
+    view.rows = rows;
+    view.cols = cols;
+    view.stencil_size = r;
+    view.neutral_value = nv;
+    view.data = in;
+    view.border_data = border_data;
+    view.border = border;
+    MSL_GRID_STRIDE(k, (size_t)rows * cols) {
+        const int row = (int)(k / (size_t)cols), col = (int)(k % (size_t)cols);
+        out[k] = f.F::operator()(row, col, &view, cols, rows);
     }
 }
 
@@ -288,25 +449,68 @@ __global__ __launch_bounds__(kThreads) void rotate_kernel(T *out, const T *in, i
     }
 }
 
-// 3-D: a 256-thread block (32 x 8) computes a 32 x 8 x 8 output tile (8 z
-// cells per thread) after staging the tile plus halo in LDS.
-constexpr int kTile3X = 32;
-constexpr int kTile3Y = 8;
-constexpr int kTile3Z = 8;
+// 3-D march: a 256-thread block owns a 64 x 8 column tile of the cube (two
+// outputs per thread per plane) and a segment of planes.  At each step it
+// stages kMarch3K planes of the window (tile plus r halo cells in x and y, nv
+// off the cube) into an LDS ring of H >= 2r + 2*kMarch3K planes, the next
+// step's planes being requested in registers meanwhile; after one barrier it
+// computes the kMarch3K output planes whose 2r+1 window planes are staged.
+constexpr int kM3X = 64;
+constexpr int kM3Y = 8;
+constexpr int kMarch3K = 2;
+constexpr int kM3Stage = 4;  // staged cells per thread per plane (r <= 3)
 
-inline size_t stencil3d_lds_bytes(int r, size_t elem) {
-    return (size_t)(kTile3X + 2 * r) * (kTile3Y + 2 * r) * (kTile3Z + 2 * r) * elem;
+inline int march3d_ring_planes(int r) { return pow2_at_least(2 * r + 2 * kMarch3K); }
+
+inline size_t march3d_lds_bytes(int r, size_t elem) {
+    const size_t plane = (size_t)(kM3X + 2 * r) * (kM3Y + 2 * r);
+    if (plane > (size_t)kM3Stage * kThreads) return (size_t)-1;
+    return (size_t)march3d_ring_planes(r) * plane * elem;
+}
+
+struct March3D {
+    int xtiles, ytiles, zseg, blocks;
+};
+
+inline March3D march3d_shape(int cols, int rows, int depth) {
+    March3D m;
+    m.xtiles = (cols + kM3X - 1) / kM3X;
+    m.ytiles = (rows + kM3Y - 1) / kM3Y;
+    const long cols2 = (long)m.xtiles * m.ytiles;
+    const long target = (long)num_cus() * 4;
+    long zsegs = (target + cols2 - 1) / cols2;
+    if (zsegs < 1) zsegs = 1;
+    int zseg = (int)((depth + zsegs - 1) / zsegs);
+    if (zseg < 4) zseg = 4;
+    m.zseg = zseg;
+    m.blocks = m.xtiles * m.ytiles * ((depth + zseg - 1) / zseg);
+    return m;
 }
 
 template <typename T, T (*f)(const PLCube<T> &, int, int, int)>
 __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *in, int cols, int rows, int depth,
-                                                              int r, T nv, int tiles_x, int tiles_y, int use_lds) {
+                                                              int r, T nv, int xtiles, int ytiles, int zseg,
+                                                              int mask) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    T *tile = reinterpret_cast<T *>(smem_raw);
-    const int bx = blockIdx.x % tiles_x;
-    const int rest = blockIdx.x / tiles_x;
-    const int by = rest % tiles_y, bz = rest / tiles_y;
-    const int x0 = bx * kTile3X, y0 = by * kTile3Y, z0 = bz * kTile3Z;
+    T *ring = reinterpret_cast<T *>(smem_raw);
+    const int bx = blockIdx.x % xtiles;
+    const int rest = blockIdx.x / xtiles;
+    const int x0 = bx * kM3X, y0 = (rest % ytiles) * kM3Y;
+    const int z0 = (rest / ytiles) * zseg, z1 = min(z0 + zseg, depth);
+    const int WX = kM3X + 2 * r, WY = kM3Y + 2 * r, WE = WX * WY;
+    const int t = threadIdx.x;
+    // staged cells of this thread: e = t + k*256, (ex, ey) in the window
+    int ex[kM3Stage], ey[kM3Stage];
+#pragma unroll
+    for (int k = 0; k < kM3Stage; k++) {
+        const int e = min(t + k * kThreads, WE - 1);
+        ex[k] = x0 - r + e % WX;
+        ey[k] = y0 - r + e / WX;
+    }
+    auto load = [&](int x, int y, int z) -> T {
+        const int xc = min(max(x, 0), cols - 1), yc = min(max(y, 0), rows - 1), zc = min(max(z, 0), depth - 1);
+        return in[((size_t)zc * rows + yc) * cols + xc];
+    };
     PLCube<T> view;
     view.width = cols;
     view.height = rows;
@@ -314,29 +518,67 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
     view.stencilSize = r;
     view.neutralValue = nv;
     view.data = in;
-    if (use_lds) {
-        const int tw = kTile3X + 2 * r, th = kTile3Y + 2 * r, td = kTile3Z + 2 * r;
-        for (int idx = threadIdx.x; idx < tw * th * td; idx += kThreads) {
-            const int lx = idx % tw, ly = (idx / tw) % th, lz = idx / (tw * th);
-            const int gx = x0 - r + lx, gy = y0 - r + ly, gz = z0 - r + lz;
-            tile[idx] = ((unsigned)gx < (unsigned)cols && (unsigned)gy < (unsigned)rows && (unsigned)gz < (unsigned)depth)
-                            ? in[((size_t)gz * rows + gy) * cols + gx]
-                            : nv;
+    view.tile = ring;
+    view.tileX = x0 - r;
+    view.tileY = y0 - r;
+    view.tileW = WX;
+    view.tileH = WY;
+    view.tileD = 2 * r + 1;
+    view.planeMask = mask;
+    const int ox = x0 + (t % kM3X), oy = y0 + t / kM3X;
+    const int total = (z1 - z0) + 2 * r;
+    T q[kMarch3K][kM3Stage];
+#pragma unroll
+    for (int d = 0; d < kMarch3K; d++)
+#pragma unroll
+        for (int k = 0; k < kM3Stage; k++) q[d][k] = load(ex[k], ey[k], z0 - r + d);
+    for (int sk = 0; sk < total; sk += kMarch3K) {  // uniform across the block
+        const int a = z0 - r + sk;                   // first plane staged at this step
+#pragma unroll
+        for (int d = 0; d < kMarch3K; d++) {
+            const int zz = a + d;
+            T *plane = ring + (zz & mask) * WE;
+#pragma unroll
+            for (int k = 0; k < kM3Stage; k++) {
+                const int e = t + k * kThreads;
+                const bool in_cube =
+                    (unsigned)ex[k] < (unsigned)cols && (unsigned)ey[k] < (unsigned)rows && (unsigned)zz < (unsigned)depth;
+                if (e < WE) plane[e] = in_cube ? q[d][k] : nv;
+                q[d][k] = load(ex[k], ey[k], zz + kMarch3K);
+            }
         }
         __syncthreads();
-        view.tile = tile;
-        view.tileX = x0 - r;
-        view.tileY = y0 - r;
-        view.tileZ = z0 - r;
-        view.tileW = tw;
-        view.tileH = th;
-        view.tileD = td;
-    }
-    const int x = x0 + (threadIdx.x & 31), y = y0 + (threadIdx.x >> 5);
 #pragma unroll
-    for (int k = 0; k < kTile3Z; k++) {
-        const int z = z0 + k;
-        if (x < cols && y < rows && z < depth) out[((size_t)z * rows + y) * cols + x] = f(view, x, y, z);
+        for (int d = 0; d < kMarch3K; d++) {
+            const int zo = a - r + d;  // its window planes a-2r+d .. a+d are staged
+            if (zo >= z0 && zo < z1) {
+                PLCube<T> v = view;
+                v.tileZ = zo - r;
+#pragma unroll
+                for (int j = 0; j < kM3Y / 4; j++) {
+                    const int y = oy + 4 * j;
+                    if (ox < cols && y < rows) out[((size_t)zo * rows + y) * cols + ox] = f(v, ox, y, zo);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, T (*f)(const PLCube<T> &, int, int, int)>
+__global__ __launch_bounds__(kThreads) void dc_stencil_global_kernel(T *out, const T *in, int cols, int rows,
+                                                                     int depth, int r, T nv) {
+    PLCube<T> view;
+    view.width = cols;
+    view.height = rows;
+    view.depth = depth;
+    view.stencilSize = r;
+    view.neutralValue = nv;
+    view.data = in;
+    MSL_GRID_STRIDE(k, (size_t)cols * rows * depth) {
+        const size_t plane = (size_t)cols * rows;
+        const int z = (int)(k / plane);
+        const size_t rem = k - (size_t)z * plane;
+        out[k] = f(view, (int)(rem % cols), (int)(rem / cols), z);
     }
 }
 

@@ -99,12 +99,16 @@ public:
         this->updateDevice();
         if (this->n == 0) return;
         const int r = (int)stencilSize;
-        const int tiles_x = (ncol + detail::kTileX - 1) / detail::kTileX;
-        const int tiles_y = (nrow + detail::kTileY - 1) / detail::kTileY;
-        const size_t lds = detail::stencil2d_lds_bytes(r, sizeof(T));
-        const int use_lds = lds <= detail::kMaxLdsBytes;
-        detail::dm_stencil_kernel<T, f><<<tiles_x * tiles_y, detail::kThreads, use_lds ? lds : 0, detail::stream()>>>(
-            result.d_data, this->d_data, nrow, ncol, r, neutralValue, tiles_x, use_lds);
+        const size_t lds = detail::march2d_lds_bytes(r, sizeof(T));
+        if (lds <= detail::kMaxLdsBytes) {
+            const detail::March2D m = detail::march2d_shape(nrow, ncol);
+            const int mask = detail::march2d_ring_rows(r) - 1;
+            detail::dm_stencil_kernel<T, f><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+                result.d_data, this->d_data, nrow, ncol, r, neutralValue, m.strips_x, m.seg, mask);
+        } else {
+            detail::dm_stencil_global_kernel<T, f><<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                result.d_data, this->d_data, nrow, ncol, r, neutralValue);
+        }
         detail::check_launch("DM::mapStencil");
         result.deviceWritten();
     }
@@ -250,13 +254,18 @@ private:
         this->updateDevice();
         if (this->n == 0) return;
         const int r = f.getStencilSize();
-        const int tiles_x = (ncol + detail::kTileX - 1) / detail::kTileX;
-        const int tiles_y = (nrow + detail::kTileY - 1) / detail::kTileY;
-        const size_t lds = detail::stencil2d_lds_bytes(r, sizeof(T));
-        const int use_lds = lds <= detail::kMaxLdsBytes;
-        detail::dm_stencil_mm_kernel<T, T2, F>
-            <<<tiles_x * tiles_y, detail::kThreads, use_lds ? lds : 0, detail::stream()>>>(
-                result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, tiles_x, use_lds, f);
+        const size_t lds = detail::march2d_lds_bytes(r, sizeof(T));
+        if (lds <= detail::kMaxLdsBytes) {
+            const detail::March2D m = detail::march2d_shape(nrow, ncol);
+            const int mask = detail::march2d_ring_rows(r) - 1;
+            detail::dm_stencil_mm_kernel<T, T2, F><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+                result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, m.strips_x, m.seg, mask,
+                f);
+        } else {
+            detail::dm_stencil_mm_global_kernel<T, T2, F>
+                <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                    result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, f);
+        }
         detail::check_launch("DM::mapStencilMM");
         result.setCpuMemoryInSync(false);
     }

@@ -148,7 +148,8 @@ public:
         const int blocks = detail::grid_for(n, 4);
         ensureFoldScratch(blocks);
         detail::fold_stage1_kernel<T, FoldFunctor>
-            <<<blocks, detail::kThreads, 0, detail::stream()>>>(d_data, (size_t)n, d_part, d_has, f);
+            <<<blocks, detail::kThreads, 0, detail::stream()>>>(d_data, (size_t)n, d_part, d_has, f,
+                                                                detail::aligned16(d_data));
         detail::check_launch("DS::fold");
         detail::fold_stage2_kernel<T, FoldFunctor>
             <<<1, detail::kThreads, 0, detail::stream()>>>(d_part, d_has, blocks, d_part + blocks, d_has + blocks, f);
@@ -369,15 +370,15 @@ protected:
     template <typename F>
     void launch_map(T *out, const T *in, F &f) {
         if (n == 0) return;
-        detail::map_kernel<T, T, F><<<detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(out, in, (size_t)n,
-                                                                                                  f);
+        detail::map_kernel<T, T, F><<<detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(
+            out, in, (size_t)n, f, detail::aligned16(out) && detail::aligned16(in));
         detail::check_launch("DS::map");
     }
     template <typename A, typename B, typename F>
     void launch_zip(T *out, const A *a, const B *b, F &f) {
         if (n == 0) return;
-        detail::zip_kernel<A, B, T, F><<<detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(out, a, b,
-                                                                                                     (size_t)n, f);
+        detail::zip_kernel<A, B, T, F><<<detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(
+            out, a, b, (size_t)n, f, detail::aligned16(out) && detail::aligned16(a) && detail::aligned16(b));
         detail::check_launch("DS::zip");
     }
 

@@ -24,6 +24,7 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "msl_hip.h"
@@ -140,6 +141,19 @@ inline int num_cus() {
 inline void check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw RuntimeError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Reads one element of a view's LDS window.  In device code the pointer is
+// re-typed as an LDS (address space 3) pointer: a generic pointer would let
+// the compiler merge the window read with the view's global-memory fallback
+// into one flat load per access.  Class types keep the generic read (their
+// copy constructors take generic references).
+template <typename T>
+MSL_USERFUNC __forceinline__ T lds_read(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_arithmetic<T>::value) return *(const __attribute__((address_space(3))) T *)p;
+#endif
+    return *p;
 }
 
 struct Timing {

@@ -4,10 +4,11 @@
  * (include/nplmatrix.h:56-168): operator()(x, y) with x the column and y the
  * row returns the neutral value outside [0,width) x [0,height).
  *
- * On the device the view resolves reads from the workgroup's LDS tile (the
- * output tile plus a halo of the stencil radius) when the cell is inside it,
- * else from global memory, so functors that read beyond the declared radius
- * still get correct values.
+ * On the device the view first looks in the workgroup's LDS window (the
+ * 2r+1 rows around the output row, plus r halo columns each side, staged
+ * with nv outside the grid; row y lives in ring slot y & rowMask), and falls
+ * back to the neutral value or global memory outside the window, so functors
+ * that read beyond the declared radius still get correct values.
  */
 #pragma once
 
@@ -27,16 +28,17 @@ public:
     int stencilSize = 0;  // declared radius
     T neutralValue{};
     const T *data = nullptr;  // device grid, row-major
-    // LDS tile: cells [tileX, tileX+tileW) x [tileY, tileY+tileH), row pitch tileW
+    // LDS window: columns [tileX, tileX+tileW), rows [tileY, tileY+tileH);
+    // row y is stored at tile + (y & rowMask) * tileW
     const T *tile = nullptr;
-    int tileX = 0, tileY = 0, tileW = 0, tileH = 0;
+    int tileX = 0, tileY = 0, tileW = 0, tileH = 0, rowMask = 0;
 
     MSL_USERFUNC NPLMatrix() {}
 
-    MSL_USERFUNC const T &operator()(int x, int y) const {
-        if ((unsigned)x >= (unsigned)width || (unsigned)y >= (unsigned)height) return neutralValue;
+    MSL_USERFUNC T operator()(int x, int y) const {
         const int lx = x - tileX, ly = y - tileY;
-        if ((unsigned)lx < (unsigned)tileW && (unsigned)ly < (unsigned)tileH) return tile[ly * tileW + lx];
+        if ((unsigned)lx < (unsigned)tileW && (unsigned)ly < (unsigned)tileH) return detail::lds_read(tile + (y & rowMask) * tileW + lx);
+        if ((unsigned)x >= (unsigned)width || (unsigned)y >= (unsigned)height) return neutralValue;
         return data[(size_t)y * width + x];
     }
 

@@ -28,19 +28,26 @@ public:
     const T *data = nullptr;        // device grid, row-major
     const T *border_data = nullptr;  // optional (rows+2b) x (cols+2b) padded values
     int border = 0;
-    const T *tile = nullptr;  // LDS tile
-    int tileRow = 0, tileCol = 0, tileRows = 0, tileCols = 0;
+    // LDS window: columns [tileCol, tileCol+tileCols), rows [tileRow,
+    // tileRow+tileRows), staged with the values get() would return (grid,
+    // border band or neutral value); row r at tile + (r & rowMask)*tileCols
+    const T *tile = nullptr;
+    int tileRow = 0, tileCol = 0, tileRows = 0, tileCols = 0, rowMask = 0;
 
     MSL_USERFUNC PLMatrix() {}
 
+    // Value of a cell outside the grid: the border band (if any), else nv.
+    MSL_USERFUNC T outside(int row, int col) const {
+        if (border_data && row >= -border && row < rows + border && col >= -border && col < cols + border)
+            return border_data[(size_t)(row + border) * (cols + 2 * border) + (col + border)];
+        return neutral_value;
+    }
+
     MSL_USERFUNC T get(int row, int col) const {
-        if ((unsigned)row >= (unsigned)rows || (unsigned)col >= (unsigned)cols) {
-            if (border_data && row >= -border && row < rows + border && col >= -border && col < cols + border)
-                return border_data[(size_t)(row + border) * (cols + 2 * border) + (col + border)];
-            return neutral_value;
-        }
         const int lr = row - tileRow, lc = col - tileCol;
-        if ((unsigned)lr < (unsigned)tileRows && (unsigned)lc < (unsigned)tileCols) return tile[lr * tileCols + lc];
+        if ((unsigned)lr < (unsigned)tileRows && (unsigned)lc < (unsigned)tileCols)
+            return detail::lds_read(tile + (row & rowMask) * tileCols + lc);
+        if ((unsigned)row >= (unsigned)rows || (unsigned)col >= (unsigned)cols) return outside(row, col);
         return data[(size_t)row * cols + col];
     }
 
