@@ -77,16 +77,27 @@ def main():
     inb = C.c_int()
     ms = C.c_float()
 
-    def timed(fn, reps):
+    # ---- per-kernel timings: each launch timed alone with events on the
+    # library stream, after a fold that reads 1 GiB and so evicts the grids
+    # from the 256 MiB Infinity Cache (non-temporal stores, as in the fills,
+    # do not allocate there and would leave the inputs cached): every launch
+    # reads its inputs from HBM
+    nflush = 1 << 28
+    flush = DeviceArray(nflush, np.float32)
+    call("msl_fill_f32", flush.ptr, nflush, 0.0)
+
+    def timed_cold(fn, reps):
         for _ in range(3):  # first launches load the code object / grow the workspace
             fn()
-        call("msl_timer_start")
+        total = 0.0
         for _ in range(reps):
+            call("msl_fold_sum_f32", flush.ptr, nflush, d_res.ptr)
+            call("msl_timer_start")
             fn()
-        call("msl_timer_stop", C.byref(ms))
-        return ms.value / reps
+            call("msl_timer_stop", C.byref(ms))
+            total += ms.value
+        return total / reps
 
-    # ---- per-kernel timings (average launch duration on the library stream)
     def fresh():
         call("msl_fill_uniform_f32", a.ptr, n, SEED, scale)
         call("msl_fill_uniform_f32", b.ptr, n, SEED + 1, scale)
@@ -95,19 +106,20 @@ def main():
     hip.synchronize()
     reps = 20
     kernel_ms = {}
-    kernel_ms["map"] = timed(lambda: call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0), reps)
-    kernel_ms["zip"] = timed(lambda: call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n), reps)
-    kernel_ms["mapindex"] = timed(lambda: call("msl_mapindex_lin_f32", c.ptr, ROWS, COLS), reps)
-    kernel_ms["fill"] = timed(lambda: call("msl_fill_f32", c.ptr, n, 1.0), reps)
-    kernel_ms["fold"] = timed(lambda: call("msl_fold_sum_f32", a.ptr, n, d_res.ptr), reps)
+    kernel_ms["map"] = timed_cold(lambda: call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0), reps)
+    kernel_ms["zip"] = timed_cold(lambda: call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n), reps)
+    kernel_ms["mapindex"] = timed_cold(lambda: call("msl_mapindex_lin_f32", c.ptr, ROWS, COLS), reps)
+    kernel_ms["fill"] = timed_cold(lambda: call("msl_fill_f32", c.ptr, n, 1.0), reps)
+    kernel_ms["fold"] = timed_cold(lambda: call("msl_fold_sum_f32", a.ptr, n, d_res.ptr), reps)
     for r in RADII:
-        kernel_ms[f"blur_r{r}"] = timed(lambda: call("msl_stencil2d_mean_f32", c.ptr, a.ptr, ROWS, COLS, r, 0.0), reps)
+        kernel_ms[f"blur_r{r}"] = timed_cold(lambda: call("msl_stencil2d_mean_f32", c.ptr, a.ptr, ROWS, COLS, r, 0.0), reps)
     kernel_bytes = {"map": 8 * n, "zip": 12 * n, "mapindex": 4 * n, "fill": 4 * n, "fold": 4 * n,
                     **{f"blur_r{r}": 8 * n for r in RADII}}
     kernels = {k: {"ms": round(v, 5), "GBs": round(kernel_bytes[k] / (v * 1e-3) / 1e9, 1),
                    "pct_hbm_peak": round(100 * kernel_bytes[k] / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)}
                for k, v in kernel_ms.items()}
     log(f"[bench] kernels {kernels}")
+    flush.free()
 
     # ---- headline step ------------------------------------------------------
     def step():
@@ -159,7 +171,7 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm",
-                "kernel": f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0> (8192x8192 ping-pong, 8 B/cell)",
+                "kernel": f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true> (8192x8192 ping-pong, 8 B/cell)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": 8 * n, "avg_launch_ms": round(launch_ms, 5)}
@@ -184,6 +196,8 @@ def main():
         "roofline": roofline,
         "per_radius": per_radius,
         "kernels": kernels,
+        "kernels_note": "each launch timed alone (HIP events on the library stream) after a 1 GiB read that "
+                        "evicts the 256 MiB Infinity Cache; blur kernels write a third grid",
         "device": info,
     }
 

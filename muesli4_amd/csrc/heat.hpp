@@ -47,7 +47,7 @@ __device__ __forceinline__ float from_right(float v, float edge) {
 // March the tile (x0 = 64 * xtile, rows y0 .. y0+kTY-1) over planes
 // z0 .. z1-1.  kResidual: fold max |v - in| into *dmax.  kSum: push every
 // stored value into *sum.
-template <bool kResidual, bool kSum>
+template <bool kResidual, bool kSum, bool NT = true>
 __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float *__restrict__ in, int nx, int ny,
                                             int nz, float nv, int x0, int y0, int z0, int z1, float *dmax,
                                             LaneSum *sum) {
@@ -100,7 +100,11 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
                 v = __fdiv_rn(v, 6.0f);
                 const int y = y0 + j;
                 if (z < z1 && x < nx && y < ny) {
-                    __builtin_nontemporal_store(v, out + ((size_t)z * ny + y) * nx + x);
+                    float *o = out + ((size_t)z * ny + y) * nx + x;
+                    if constexpr (NT)
+                        __builtin_nontemporal_store(v, o);
+                    else
+                        *o = v;
                     if constexpr (kResidual) {
                         const float d = fabsf(__fsub_rn(v, c));
                         *dmax = d > *dmax ? d : *dmax;

@@ -24,7 +24,7 @@ constexpr int kStripW = StripConsts<1, 4>::SW;  // columns per wave strip
 // One wave per (strip, segment) item, four independent waves per 256-thread
 // workgroup; each wave stages rows in its own LDS region, so only
 // wave-level ordering is needed (no workgroup barrier).
-template <typename Pol, int MODE>
+template <typename Pol, int MODE, bool NT = true>
 __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__restrict__ out,
                                                             const typename Pol::T *__restrict__ in, int rows, int cols,
                                                             typename Pol::T nv, int seg, int strips_x, int nitems,
@@ -39,8 +39,8 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     const int sx = item % strips_x;
     const int sy = item / strips_x;
     const int y0 = sy * seg;
-    strip_march<Pol, 4, MODE>(out, in, rows, cols, nv, sx * kStripW, y0, min(y0 + seg, rows), pol, lds[wave], nullptr,
-                              sy & 1);
+    strip_march<Pol, 4, MODE, false, 0, NT>(out, in, rows, cols, nv, sx * kStripW, y0, min(y0 + seg, rows), pol,
+                                            lds[wave], nullptr, sy & 1);
 }
 
 // Row access mode of a 4-column-per-lane march over these buffers.
@@ -79,9 +79,12 @@ int launch_strip(typename Pol::T *out, const typename Pol::T *in, int rows, int 
     StripShape s = strip_shape(rows, cols, seg);
     const int blocks = (s.nitems + 3) / 4;
     const int mode = row_mode(in, out, cols);
-    if (mode == kAligned)
-        strip_stencil_kernel<Pol, kAligned><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
-                                                                                s.strips_x, s.nitems, pol);
+    if (mode == kAligned && stream_stores((size_t)rows * cols * sizeof(*out)))
+        strip_stencil_kernel<Pol, kAligned, true><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
+                                                                                      s.strips_x, s.nitems, pol);
+    else if (mode == kAligned)
+        strip_stencil_kernel<Pol, kAligned, false><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
+                                                                                       s.strips_x, s.nitems, pol);
     else if (mode == kUnaligned)
         strip_stencil_kernel<Pol, kUnaligned><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
                                                                                   s.strips_x, s.nitems, pol);

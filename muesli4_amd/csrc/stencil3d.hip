@@ -21,7 +21,7 @@ namespace mslhip {
 
 // One wave per (64 x kTY column tile, z-segment); four independent waves per
 // workgroup, no LDS and no barrier.
-template <bool kResidual>
+template <bool kResidual, bool NT>
 __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, const float *__restrict__ in, int nx,
                                                     int ny, int nz, float nv, int zseg, int xtiles, int ytiles,
                                                     int nitems, float *__restrict__ maxdiff) {
@@ -34,8 +34,8 @@ __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, con
         const int yt = rest % ytiles;
         const int zs = rest / ytiles;
         const int z0 = zs * zseg;
-        heat7_march<kResidual, false>(out, in, nx, ny, nz, nv, xt * kTX, yt * kTY, z0, min(z0 + zseg, nz), &dmax,
-                                      nullptr);
+        heat7_march<kResidual, false, NT>(out, in, nx, ny, nz, nv, xt * kTX, yt * kTY, z0, min(z0 + zseg, nz), &dmax,
+                                          nullptr);
     }
     if (kResidual) {
         __shared__ float wmax[4];
@@ -83,12 +83,20 @@ int heat_step(float *out, const float *in, int nx, int ny, int nz, float nv, flo
     if (nx == 0 || ny == 0 || nz == 0) return MSL_OK;
     MSL_REQUIRE(out && in && out != in, "msl_heat7_f32: null or aliased buffers");
     HeatShape s = heat_shape(nx, ny, nz);
-    if (maxdiff)
-        heat7_kernel<true><<<(s.nitems + 3) / 4, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
-                                                                s.nitems, maxdiff);
+    const bool nt = stream_stores((size_t)nx * ny * nz * sizeof(float));
+    const int blocks = (s.nitems + 3) / 4;
+    if (maxdiff && nt)
+        heat7_kernel<true, true><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
+                                                                   s.nitems, maxdiff);
+    else if (maxdiff)
+        heat7_kernel<true, false><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
+                                                                    s.nitems, maxdiff);
+    else if (nt)
+        heat7_kernel<false, true><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
+                                                                    s.nitems, nullptr);
     else
-        heat7_kernel<false><<<(s.nitems + 3) / 4, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles, s.ytiles,
-                                                                 s.nitems, nullptr);
+        heat7_kernel<false, false><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles,
+                                                                     s.ytiles, s.nitems, nullptr);
     return check_launch("msl_heat7_f32");
 }
 

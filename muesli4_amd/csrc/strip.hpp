@@ -141,12 +141,18 @@ __device__ __forceinline__ VecN<T, C> fix_vec(const VecN<T, C> &v, int yy, int c
     return q;
 }
 
-// Stores the lane's C outputs of row orow starting at column x0.
-template <typename T, int C, int MODE>
+// Stores the lane's C outputs of row orow starting at column x0 (aligned
+// rows: non-temporal stores when NT, see stream_stores).
+template <typename T, int C, int MODE, bool NT>
 __device__ __forceinline__ void write_vec(T *__restrict__ orow, int x0, int cols, const VecN<T, C> &o) {
     typedef typename NativeVecN<T, C>::type V;
     if constexpr (MODE == kAligned) {
-        if (x0 < cols) __builtin_nontemporal_store(to_native<T, C>(o), reinterpret_cast<V *>(orow + x0));
+        if (x0 < cols) {
+            if constexpr (NT)
+                __builtin_nontemporal_store(to_native<T, C>(o), reinterpret_cast<V *>(orow + x0));
+            else
+                *reinterpret_cast<V *>(orow + x0) = to_native<T, C>(o);
+        }
     } else if constexpr (MODE == kUnaligned) {
         if (x0 + C <= cols) {
             const V x = to_native<T, C>(o);
@@ -254,7 +260,7 @@ struct LaneSum {
 // register index is a compile-time constant.  Row t+D is requested when row t
 // is staged (DQ > 0 overrides the default prefetch depth).  With kSum, every
 // stored value is also pushed into *sum.
-template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP>
+template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP, bool NT>
 __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ out,
                                                 const typename Pol::T *__restrict__ in, int rows, int cols,
                                                 typename Pol::T nv, int xs, int y0, int yend, const Pol &pol,
@@ -337,7 +343,7 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
                     if constexpr (Pol::kCenter) centre = cring[(u + PER - R) % PER].v[c];
                     o.v[c] = pol.finish(acc, centre);
                 }
-                write_vec<T, C, MODE>(out + (size_t)y * cols, x0, cols, o);
+                write_vec<T, C, MODE, NT>(out + (size_t)y * cols, x0, cols, o);
                 if constexpr (kSum) {
 #pragma unroll
                     for (int c = 0; c < C; c++)
@@ -354,15 +360,15 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 // adjacent segments makes both readers of the 2R halo rows between them
 // read those rows at the same time (both at the start or both at the end
 // of their marches), so the second read is served by L2.
-template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0>
+template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0, bool NT = true>
 __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
                                             int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
                                             const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW],
                                             LaneSum *sum = nullptr, int up = 0) {
     if (up)
-        strip_march_dir<Pol, C, MODE, kSum, DQ, true>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
+        strip_march_dir<Pol, C, MODE, kSum, DQ, true, NT>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
     else
-        strip_march_dir<Pol, C, MODE, kSum, DQ, false>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
+        strip_march_dir<Pol, C, MODE, kSum, DQ, false, NT>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
 }
 
 }  // namespace mslhip

@@ -216,9 +216,11 @@ def test_mean_blur_rejects_bad_radius():
         call("msl_stencil2d_mean_f32", a.ptr, a.ptr, 4, 4, 1, 0.0)
 
 
-@pytest.mark.parametrize("r", [1, 5, 10])
-def test_mean_blur_8192_single_step(r):
-    rows = cols = 8192 if r == 1 else 2048
+# Grids above 128 MiB take the non-temporal-store kernels, smaller ones the
+# ordinary-store kernels; both are covered.
+@pytest.mark.parametrize("r,rows,cols", [(1, 8192, 8192), (5, 5632, 6144), (10, 5632, 6144), (5, 2048, 2048),
+                                         (10, 2048, 2048)])
+def test_mean_blur_8192_single_step(r, rows, cols):
     note(f"mean full r={r} {rows}x{cols}")
     n = rows * cols
     a = dev_uniform(n, 5)
@@ -309,6 +311,22 @@ def test_heat7_512_single_step():
     out = DeviceArray(shape, np.float32)
     call("msl_heat7_f32", out.ptr, a.ptr, 512, 512, 512, 0.0)
     np.testing.assert_array_equal(out.to_host(), R.heat7_f32(R.fill_uniform_f32(n, 41, R.SCALE_UNIT).reshape(shape)))
+
+
+def test_heat7_residual_above_store_threshold():
+    """A 138 MiB cube: the residual kernel with non-temporal stores."""
+    shape = (336, 320, 336)
+    nz, ny, nx = shape
+    note(f"heat7 residual {shape}")
+    n = nx * ny * nz
+    a = dev_uniform(n, 43, R.SCALE_UNIT)
+    out = DeviceArray(shape, np.float32)
+    md = DeviceArray(1, np.float32)
+    call("msl_fill_f32", md.ptr, 1, 0.0)
+    call("msl_heat7_residual_f32", out.ptr, a.ptr, nx, ny, nz, 0.0, md.ptr)
+    ref, m = R.heat7_residual_f32(R.fill_uniform_f32(n, 43, R.SCALE_UNIT).reshape(shape), 0.0)
+    np.testing.assert_array_equal(out.to_host(), ref)
+    assert float(md.to_host()[0]) == m
 
 
 def test_heat7_iterate_with_residuals():
