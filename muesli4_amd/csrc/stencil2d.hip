@@ -23,15 +23,17 @@ constexpr int kStripW = StripConsts<1, 4>::SW;  // columns per wave strip
 
 // One wave per (strip, segment) item, four independent waves per 256-thread
 // workgroup; each wave stages rows in its own LDS region, so only
-// wave-level ordering is needed (no workgroup barrier).
-template <typename Pol, int MODE, bool NT = true>
+// wave-level ordering is needed (no workgroup barrier).  C columns per lane:
+// 4 (256-column strips) in general, 1 (64-column strips) for small grids,
+// where four times as many strips keep more waves busy.
+template <typename Pol, int MODE, bool NT = true, int C = 4>
 __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__restrict__ out,
                                                             const typename Pol::T *__restrict__ in, int rows, int cols,
                                                             typename Pol::T nv, int seg, int strips_x, int nitems,
                                                             Pol pol) {
     typedef typename Pol::T T;
     constexpr int R = Pol::R;
-    __shared__ __attribute__((aligned(16))) T lds[4][2][StripConsts<R, 4>::LDSW];
+    __shared__ __attribute__((aligned(16))) T lds[4][2][StripConsts<R, C>::LDSW];
 
     const int wave = threadIdx.x >> 6;
     const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
@@ -39,8 +41,8 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     const int sx = item % strips_x;
     const int sy = item / strips_x;
     const int y0 = sy * seg;
-    strip_march<Pol, 4, MODE, false, 0, NT>(out, in, rows, cols, nv, sx * kStripW, y0, min(y0 + seg, rows), pol,
-                                            lds[wave], nullptr, sy & 1);
+    strip_march<Pol, C, MODE, false, 0, NT>(out, in, rows, cols, nv, sx * StripConsts<R, C>::SW, y0,
+                                            min(y0 + seg, rows), pol, lds[wave], nullptr, sy & 1);
 }
 
 // Row access mode of a 4-column-per-lane march over these buffers.
@@ -53,19 +55,33 @@ inline int row_mode(const T *in, const T *out, int cols) {
 }
 
 struct StripShape {
-    int seg, strips_x, segs, nitems;
+    int seg, strips_x, segs, nitems, c;
 };
 
-// Segment length: about 16 waves per CU.
-inline StripShape strip_shape(int rows, int cols, int seg) {
+// Columns per lane and segment length (measured on 512^2 .. 8192^2 grids,
+// r = 1, 5, 10).  A grid whose 256-column strips cannot give 8 waves per CU
+// at 8-row segments marches 64-column strips (one column per lane) instead.
+// Segments aim at 8 waves per CU with 256-column strips on grids that stay
+// in the Infinity Cache, 16 on grids streamed from HBM (longer load latency
+// to cover), and 32 with 64-column strips, which carry a quarter of the
+// loads per row each.  They stay at least 4, 8 or 16 rows long for R <= 2,
+// <= 6 and larger, so that the 2R halo rows read by each segment stay a
+// modest fraction of its rows.
+inline StripShape strip_shape(int rows, int cols, int r, int seg, bool streamed) {
     StripShape s;
-    s.strips_x = (cols + kStripW - 1) / kStripW;
+    const long cus = ctx().num_cus;
+    const long strips4 = (cols + kStripW - 1) / kStripW;
+    s.c = tune_int("MSL_STRIP_C", 0);
+    if (s.c != 1 && s.c != 4) s.c = strips4 * ((rows + 7) / 8) < 8 * cus ? 1 : 4;
+    const int sw = 64 * s.c;
+    s.strips_x = (cols + sw - 1) / sw;
     if (seg <= 0) seg = tune_int("MSL_STRIP_SEG", 0);
     if (seg <= 0) {
-        long target = (long)ctx().num_cus * 16;
-        long segs = (target + s.strips_x - 1) / s.strips_x;
+        const long target = cus * (s.c == 1 ? 32 : (streamed ? 16 : 8));
+        const long segs = (target + s.strips_x - 1) / s.strips_x;
         seg = (int)((rows + segs - 1) / segs);
-        if (seg < 8) seg = 8;
+        const int seg_min = r <= 2 ? 4 : (r <= 6 ? 8 : 16);
+        if (seg < seg_min) seg = seg_min;
     }
     s.seg = seg;
     s.segs = (rows + seg - 1) / seg;
@@ -76,10 +92,17 @@ inline StripShape strip_shape(int rows, int cols, int seg) {
 template <typename Pol>
 int launch_strip(typename Pol::T *out, const typename Pol::T *in, int rows, int cols, typename Pol::T nv, Pol pol,
                  int seg, const char *name) {
-    StripShape s = strip_shape(rows, cols, seg);
+    const bool nt = stream_stores((size_t)rows * cols * sizeof(*out));
+    StripShape s = strip_shape(rows, cols, Pol::R, seg, nt);
     const int blocks = (s.nitems + 3) / 4;
     const int mode = row_mode(in, out, cols);
-    if (mode == kAligned && stream_stores((size_t)rows * cols * sizeof(*out)))
+    if (s.c == 1 && nt)
+        strip_stencil_kernel<Pol, kAligned, true, 1><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
+                                                                                         s.strips_x, s.nitems, pol);
+    else if (s.c == 1)
+        strip_stencil_kernel<Pol, kAligned, false, 1><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
+                                                                                          s.strips_x, s.nitems, pol);
+    else if (mode == kAligned && nt)
         strip_stencil_kernel<Pol, kAligned, true><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, s.seg,
                                                                                       s.strips_x, s.nitems, pol);
     else if (mode == kAligned)

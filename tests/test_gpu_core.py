@@ -268,7 +268,26 @@ def test_gol_1024_100_iterations_and_fold():
     assert s.value == int(ref.sum())
 
 
-@pytest.mark.parametrize("shape", [(3, 5), (17, 33), (100, 260)])
+# Small grids march 64-column strips; grids with enough 256-column strip
+# segments (on a 256-CU device) march 256-column strips in the aligned,
+# unaligned-row (cols % 4 != 0) and scalar (cols < 4) row modes.
+WIDE_SHAPES_2D = [(1900, 2050), (16400, 3), (64, 65536), (2048, 2048)]
+
+
+@pytest.mark.parametrize("r", [1, 3, 7, 10])
+def test_mean_blur_wide_strip_modes(r):
+    for (rows, cols) in WIDE_SHAPES_2D:
+        for nv in (0.0, 3.5):
+            note(f"mean wide r={r} {rows}x{cols} nv={nv}")
+            n = rows * cols
+            a = dev_uniform(n, 200 + r)
+            out = DeviceArray((rows, cols), np.float32)
+            call("msl_stencil2d_mean_f32", out.ptr, a.ptr, rows, cols, r, nv)
+            ref = R.stencil2d_mean_f32(R.fill_uniform_f32(n, 200 + r).reshape(rows, cols), r, nv)
+            np.testing.assert_array_equal(out.to_host(), ref, err_msg=f"r={r} shape={rows}x{cols} nv={nv}")
+
+
+@pytest.mark.parametrize("shape", [(3, 5), (17, 33), (100, 260)] + WIDE_SHAPES_2D)
 def test_gol_single_step_ragged(shape):
     rows, cols = shape
     note(f"gol {shape}")
