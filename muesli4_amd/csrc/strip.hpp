@@ -230,14 +230,15 @@ struct StripConsts {
     static constexpr int LDSW = SW + 2 * RP;        // elements per staged row
     static constexpr int HL = RP / C;               // halo vectors per side
     static constexpr int NQ = 2 * HL + 1;           // window vectors per lane
-    static constexpr int PER = 2 * R + 1;           // ring period (= loop unroll)
+    static constexpr int PER = 2 * R + 1;           // rows in one output's window
     static_assert(2 * HL <= 64, "halo wider than a wavefront");
     // Rows requested ahead of use: larger where the per-row work is too
-    // short to hide latency (the march unrolls over lcm(PER, D) rows).  With
-    // one column per lane the rows are narrow and the whole period is kept
-    // in flight.
+    // short to hide latency (see ring_slots for the unroll this implies).
+    // With one column per lane the rows are narrow and the whole window is
+    // kept in flight.
     static constexpr int D =
-        C == 1 ? PER : (R == 1 ? 6 : (PER <= 7 ? PER : (R > 10 ? 1 : (PER % 5 == 0 ? 5 : (PER % 3 == 0 ? 3 : 1)))));
+        C == 1 ? PER
+               : (R == 1 ? 6 : (PER <= 7 ? PER : (R > 10 ? 1 : (PER % 5 == 0 ? 5 : (PER % 3 == 0 || HL >= 3 ? 3 : 1)))));
 };
 
 // Sum of the values one wave stores, kept as a compensated (s, c) pair per
@@ -251,6 +252,16 @@ struct LaneSum {
         c = __fadd_rn(c, e);
     }
 };
+
+// Row-sum ring size for a window of PER rows and a prefetch depth of D rows:
+// the march loop is unrolled over lcm(ring size, D) rows so that every ring
+// and queue slot is a compile-time register index.  When neither of PER and
+// D divides the other, the ring is rounded up to a multiple of D (a few spare
+// slots instead of an unroll of PER * D rows, which would overflow the
+// instruction cache).
+constexpr int ring_slots(int PER, int D) {
+    return (D % PER == 0 || PER % D == 0) ? PER : D * ((PER + D - 1) / D);
+}
 
 // March one wave down its (strip, segment): input rows y0-R .. yend-1+R,
 // output rows y0 .. yend-1 of the strip starting at column xs.  One code
@@ -273,9 +284,10 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
     constexpr int PER = K::PER;
     constexpr int D = DQ > 0 ? DQ : K::D;
     constexpr int HL = K::HL;
-    // the loop body is unrolled over UL = lcm(PER, D) rows so that both the
-    // ring slot (t mod PER) and the queue slot (t mod D) are static
-    constexpr int UL = PER * D / gcd_c(PER, D);
+    // the loop body is unrolled over UL = lcm(P, D) rows so that both the
+    // ring slot (t mod P) and the queue slot (t mod D) are static
+    constexpr int P = ring_slots(PER, D);
+    constexpr int UL = P * D / gcd_c(P, D);
 
     const int lane = threadIdx.x & 63;
     const int x0 = xs + C * lane;
@@ -290,8 +302,8 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
     const int hx = has_halo ? (lane < HL ? xs - RP + C * lane : xs + K::SW + C * (lane - HL)) : x0;
     const int hslot = has_halo ? (lane < HL ? C * lane : RP + K::SW + C * (lane - HL)) : RP + C * lane;
 
-    VecN<T, C> ring[PER];  // row sums
-    VecN<T, C> cring[Pol::kCenter ? PER : 1];
+    VecN<T, C> ring[P];  // row sums of the last P rows (the last PER are used)
+    VecN<T, C> cring[Pol::kCenter ? P : 1];
     VecN<T, C> qc[D], qh[D];  // raw loads of rows t .. t+D-1 (centre and halo vectors)
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -303,7 +315,7 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 #pragma unroll
         for (int uu = 0; uu < UL; uu++) {
             const int t = tb + uu;
-            const int u = uu % PER;  // ring slot of row t (tb is a multiple of PER)
+            const int u = uu % P;  // ring slot of row t (tb is a multiple of P)
             const int qs = uu % D;   // queue slot of row t (tb is a multiple of D)
             T *buf = lds[t & 1];
             typedef typename NativeVecN<T, C>::type V;
@@ -338,9 +350,9 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
                     T acc = pol.colsum_init();
 #pragma unroll
                     for (int k = 0; k < PER; k++)
-                        acc = pol.acc(acc, ring[UP ? (u + PER - k) % PER : (u + 1 + k) % PER].v[c]);
+                        acc = pol.acc(acc, ring[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
                     T centre = T(0);
-                    if constexpr (Pol::kCenter) centre = cring[(u + PER - R) % PER].v[c];
+                    if constexpr (Pol::kCenter) centre = cring[(u + P - R) % P].v[c];
                     o.v[c] = pol.finish(acc, centre);
                 }
                 write_vec<T, C, MODE, NT>(out + (size_t)y * cols, x0, cols, o);
