@@ -14,14 +14,15 @@ constexpr int kKindAligned4 = 0;    // C = 4, aligned rows
 constexpr int kKindUnaligned4 = 1;  // C = 4, dword-aligned rows
 constexpr int kKindNarrow1 = 2;     // C = 1
 
-// Prefetch depth per (R, C) in the batched kernels: divisors of 2R+1 chosen
-// so that the kernel holding every specialisation stays within 168 VGPRs
-// (three 4-wave workgroups per CU); narrow strips keep the whole ring
-// period in flight.
+// Prefetch depth per (R, C) in the batched kernels: three rows beyond
+// R = 2 (the row-sum ring is rounded up to a multiple of 3 where 2R+1 is
+// not one), which keeps the kernel holding every specialisation within 168
+// VGPRs (three 4-wave workgroups per CU); narrow strips keep the whole
+// window in flight.
 template <int R, int C>
 struct BatchDepth {
     static constexpr int PER = 2 * R + 1;
-    static constexpr int D = C == 1 ? PER : (PER <= 5 ? PER : (PER % 3 == 0 ? 3 : 1));
+    static constexpr int D = C == 1 ? PER : (PER <= 5 ? PER : 3);
 };
 
 inline int width_class(int cols) {
