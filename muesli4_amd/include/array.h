@@ -1,6 +1,8 @@
 /*
- * array.h -- fixed-size value array used by example programs (name and
- * interface as in the reference include/array.h).
+ * array.h -- fixed-size value array for user functors, with the reference's
+ * name and interface (include/array.h:4-18): array<T, N> with data[N],
+ * size() and operator[] usable on the host and the device.  The ACO
+ * examples use it for cities and ant states.
  */
 #pragma once
 
@@ -11,7 +13,8 @@
 template <typename T, unsigned int N>
 struct array {
     T data[N];
-    int size() const { return N; }
-    MSL_USERFUNC T operator[](size_t n) const { return data[n]; }
-    MSL_USERFUNC T &operator[](size_t n) { return data[n]; }
+
+    [[nodiscard]] int size() const { return (int)N; }
+    MSL_USERFUNC T operator[](size_t i) const { return data[i]; }
+    MSL_USERFUNC T &operator[](size_t i) { return data[i]; }
 };

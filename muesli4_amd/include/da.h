@@ -10,6 +10,9 @@
 namespace msl {
 
 template <typename T>
+class DM;
+
+template <typename T>
 class DA : public DS<T> {
 public:
     DA() : DS<T>() {}
@@ -59,6 +62,35 @@ public:
         launch_index_zip(b.getUserFunctionData(), c.getUserFunctionData(), f);
         this->deviceWritten();
     }
+    // this[i] = f(i, &rows[i][0], this[i]): the functor sees row i of the
+    // matrix through a device pointer (src/da.cpp:256-289)
+    template <typename MapIndexFunctor, typename T2>
+    void mapIndexInPlaceDMRows(DM<T2> &rows, MapIndexFunctor &f) {
+        if (this->n > rows.getnRow()) throw detail::SizeMismatchException();
+        this->updateDevice();
+        rows.updateDevice();
+        if (this->n == 0) return;
+        detail::map_index_dmrows_kernel<T, T2, T, MapIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, rows.getUserFunctionData(), (size_t)this->n, rows.getnCol(), f);
+        detail::check_launch("DA::mapIndexInPlaceDMRows");
+        this->deviceWritten();
+    }
+    // this[i] = f(i, &rows[i][0], b[i], this[i])  (src/da.cpp:290-324)
+    template <typename MapIndexFunctor, typename T1, typename T2>
+    void zipIndexInPlaceDMRows(DA<T1> &b, DM<T2> &rows, MapIndexFunctor &f) {
+        if (this->n > rows.getnRow() || b.getSize() != this->n) throw detail::SizeMismatchException();
+        this->updateDevice();
+        b.updateDevice();
+        rows.updateDevice();
+        if (this->n == 0) return;
+        detail::zip_index_dmrows_kernel<T, T1, T2, T, MapIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, b.getUserFunctionData(), rows.getUserFunctionData(), (size_t)this->n,
+                rows.getnCol(), f);
+        detail::check_launch("DA::zipIndexInPlaceDMRows");
+        this->deviceWritten();
+    }
     // this[i] = f(this[i], b[i], c[i])  (src/da.cpp:432-459)
     template <typename T2, typename T3, typename ZipFunctor>
     void zipInPlace3(DA<T2> &b, DA<T3> &c, ZipFunctor &f) {
@@ -71,7 +103,6 @@ public:
     template <typename Functor>
     inline void permutePartition(Functor & /*f*/) {}
 
-    void showLocal(const std::string &descr) { this->show(descr); }
 
 private:
     template <typename In, typename F>

@@ -116,7 +116,14 @@ public:
     int getRows() { return nrow; }
     int getDepth() { return depth; }
 
-    void show(const std::string &descr = std::string(), int limited = 0) { DS<T>::show(descr, limited); }
+    // one row per line, planes separated by a dashed line; like the
+    // reference (src/dc.cpp:290) it leaves std::cout at precision 2
+    void show(const std::string &descr = std::string(), int limited = 0) {
+        std::cout.precision(2);
+        this->updateHost();
+        const long count = (limited > 0 && limited < this->n) ? limited : this->n;
+        printv("%s", detail::show_text(descr, this->localPartition, count, ncol, (long)ncol * nrow).c_str());
+    }
     void printnCPU() {}
 
     int nlocalRows = 0;

@@ -141,6 +141,126 @@ __global__ __launch_bounds__(kThreads) void zip_index_2d_kernel(R *out, const A 
     }
 }
 
+// ---- DM index skeletons with extra operands (reference src/dm.cpp:1757-1911,
+// src/zip_kernels.cu:46-99): per-element operands index by k, per-row
+// operands (a DA of one value per row) by the row.
+// out[k] = f(row, col, a[k], b[k], c[k])
+template <typename A, typename B, typename C, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void zip3_index_2d_kernel(R *out, const A *a, const B *b, const C *c, size_t n,
+                                                                 int cols, F f) {
+    MSL_GRID_STRIDE(k, n) {
+        const int row = (int)(k / cols), col = (int)(k % cols);
+        out[k] = f.F::operator()(row, col, a[k], b[k], c[k]);
+    }
+}
+
+// out[k] = f(row, col, a[k], b[k], v[row])
+template <typename A, typename B, typename V, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void zip_index_2d_row_kernel(R *out, const A *a, const B *b, const V *v,
+                                                                    size_t n, int cols, F f) {
+    MSL_GRID_STRIDE(k, n) {
+        const int row = (int)(k / cols), col = (int)(k % cols);
+        out[k] = f.F::operator()(row, col, a[k], b[k], v[row]);
+    }
+}
+
+// out[k] = f(row, col, a[k], v[row])
+template <typename A, typename V, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void map_index_2d_row_kernel(R *out, const A *a, const V *v, size_t n, int cols,
+                                                                    F f) {
+    MSL_GRID_STRIDE(k, n) {
+        const int row = (int)(k / cols), col = (int)(k % cols);
+        out[k] = f.F::operator()(row, col, a[k], v[row]);
+    }
+}
+
+// out[k] = f(row, col, a[k], u[row], v[row], w[row])
+template <typename A, typename U, typename V, typename W, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void zip_index_2d_rows3_kernel(R *out, const A *a, const U *u, const V *v,
+                                                                      const W *w, size_t n, int cols, F f) {
+    MSL_GRID_STRIDE(k, n) {
+        const int row = (int)(k / cols), col = (int)(k % cols);
+        out[k] = f.F::operator()(row, col, a[k], u[row], v[row], w[row]);
+    }
+}
+
+// out[k] = f(row, col, a[k], whole) -- the functor sees the whole second
+// matrix through its device pointer
+template <typename A, typename B, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void cross_zip_index_2d_kernel(R *out, const A *a, B *whole, size_t n, int cols,
+                                                                      F f) {
+    MSL_GRID_STRIDE(k, n) {
+        const int row = (int)(k / cols), col = (int)(k % cols);
+        out[k] = f.F::operator()(row, col, a[k], whole);
+    }
+}
+
+// out[row] = f(...f(f(in[row][0], in[row][1]), in[row][2])..., in[row][cols-1])
+// in a fixed tree: one wave per row, lane l folds columns l, l+64, ... in
+// order, then lane 0 folds the 64 lane results in lane order.  Any T (the
+// lane results go through LDS, not shuffles).
+template <typename T, typename F>
+__global__ __launch_bounds__(kThreads) void reduce_rows_kernel(T *out, const T *in, int rows, int cols, F f) {
+    __shared__ __attribute__((aligned(16))) unsigned char part_raw[kThreads * sizeof(T)];
+    __shared__ int has[kThreads];
+    T *part = reinterpret_cast<T *>(part_raw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = blockIdx.x * (kThreads / 64) + wave;
+    if (row < rows) {  // uniform per wave
+        const T *p = in + (size_t)row * cols;
+        T acc{};
+        int any = 0;
+        for (int c = lane; c < cols; c += 64) {
+            acc = any ? f.F::operator()(acc, p[c]) : p[c];
+            any = 1;
+        }
+        part[threadIdx.x] = acc;
+        has[threadIdx.x] = any;
+    }
+    __syncthreads();
+    if (row < rows && lane == 0) {
+        T acc = part[threadIdx.x];
+        for (int l = 1; l < 64; l++)
+            if (has[threadIdx.x + l]) acc = f.F::operator()(acc, part[threadIdx.x + l]);
+        out[row] = acc;
+    }
+}
+
+// out[row] = f(in[row][cols-1], in[row][0], ... f(in[row][1], in[row][2],
+// f(in[row][0], in[row][1], R(0)))) -- the reference's sequential order
+// (src/dm.cpp:1731-1754), one thread per row.
+template <typename T, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void reduce_two_columns_kernel(R *out, const T *in, int rows, int cols, F f) {
+    MSL_GRID_STRIDE(row, (size_t)rows) {
+        const T *p = in + row * (size_t)cols;
+        R acc = 0;
+        for (int j = 0; j < cols; j++) acc = f.F::operator()(p[j], p[(j + 1) % cols], acc);
+        out[row] = acc;
+    }
+}
+
+// m[row][col] = v[row]
+template <typename T>
+__global__ __launch_bounds__(kThreads) void set_column_kernel(T *m, const T *v, int rows, int cols, int col) {
+    MSL_GRID_STRIDE(row, (size_t)rows) m[row * (size_t)cols + col] = v[row];
+}
+
+// DA with a matrix of rows (reference src/da.cpp:256-324,
+// src/map_kernels.cu:80-86, src/zip_kernels.cu:73-79):
+// out[i] = f(i, &m[i * mcols], in[i])
+template <typename T, typename M, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void map_index_dmrows_kernel(R *out, const T *in, M *m, size_t n, int mcols,
+                                                                    F f) {
+    MSL_GRID_STRIDE(i, n) out[i] = f.F::operator()((int)i, m + i * (size_t)mcols, in[i]);
+}
+
+// out[i] = f(i, &m[i * mcols], b[i], in[i])
+template <typename T, typename B, typename M, typename R, typename F>
+__global__ __launch_bounds__(kThreads) void zip_index_dmrows_kernel(R *out, const T *in, const B *b, M *m, size_t n,
+                                                                    int mcols, F f) {
+    MSL_GRID_STRIDE(i, n) out[i] = f.F::operator()((int)i, m + i * (size_t)mcols, b[i], in[i]);
+}
+
 // DC: out[k] = f(col, row, depth, in[k])
 template <typename T, typename R, typename F>
 __global__ __launch_bounds__(kThreads) void map_index_3d_kernel(R *out, const T *in, size_t n, int cols, int rows,

@@ -92,6 +92,133 @@ public:
         this->deviceWritten();
     }
 
+    // ---- index skeletons with extra operands (src/dm.cpp:502-540,
+    // 1757-1911): per-element operands are matrices of this shape, per-row
+    // operands are DA<.> with one value per row.
+    // this[k] = f(row, col, this[k], b[k], c[k])
+    template <typename T2, typename T3, typename ZipIndexFunctor>
+    void zipIndexInPlace3(DM<T2> &b, DM<T3> &c, ZipIndexFunctor &f) {
+        checkSameShape(b);
+        checkSameShape(c);
+        this->updateDevice();
+        b.updateDevice();
+        c.updateDevice();
+        if (this->n == 0) return;
+        detail::zip3_index_2d_kernel<T, T2, T3, T, ZipIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, b.getUserFunctionData(), c.getUserFunctionData(), (size_t)this->n, ncol, f);
+        detail::check_launch("DM::zipIndexInPlace3");
+        this->deviceWritten();
+    }
+    // this[k] = f(row, col, this[k], b[k], c[row])
+    template <typename T2, typename T3, typename ZipIndexFunctor>
+    void zipIndexInPlaceMA(DM<T2> &b, DA<T3> &c, ZipIndexFunctor &f) {
+        checkSameShape(b);
+        checkRowVector(c);
+        this->updateDevice();
+        b.updateDevice();
+        c.updateDevice();
+        if (this->n == 0) return;
+        detail::zip_index_2d_row_kernel<T, T2, T3, T, ZipIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, b.getUserFunctionData(), c.getUserFunctionData(), (size_t)this->n, ncol, f);
+        detail::check_launch("DM::zipIndexInPlaceMA");
+        this->deviceWritten();
+    }
+    // this[k] = f(row, col, this[k], c[row])
+    template <typename T3, typename ZipIndexFunctor>
+    void zipIndexInPlaceA(DA<T3> &c, ZipIndexFunctor &f) {
+        checkRowVector(c);
+        this->updateDevice();
+        c.updateDevice();
+        if (this->n == 0) return;
+        detail::map_index_2d_row_kernel<T, T3, T, ZipIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, c.getUserFunctionData(), (size_t)this->n, ncol, f);
+        detail::check_launch("DM::zipIndexInPlaceA");
+        this->deviceWritten();
+    }
+    // this[k] = f(row, col, this[k], b[row], c[row], d[row])
+    template <typename T2, typename T3, typename T4, typename ZipIndexFunctor>
+    void zipIndexInPlaceAAA(DA<T2> &b, DA<T3> &c, DA<T4> &d, ZipIndexFunctor &f) {
+        checkRowVector(b);
+        checkRowVector(c);
+        checkRowVector(d);
+        this->updateDevice();
+        b.updateDevice();
+        c.updateDevice();
+        d.updateDevice();
+        if (this->n == 0) return;
+        detail::zip_index_2d_rows3_kernel<T, T2, T3, T4, T, ZipIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, b.getUserFunctionData(), c.getUserFunctionData(), d.getUserFunctionData(),
+                (size_t)this->n, ncol, f);
+        detail::check_launch("DM::zipIndexInPlaceAAA");
+        this->deviceWritten();
+    }
+    // this[k] = f(row, col, this[k], b) with b the device pointer of the
+    // whole matrix b (the reference CPU form, src/dm.cpp:525-532; its GPU
+    // kernel calls f(row, col, this, b) without storing a result)
+    template <typename T2, typename ZipIndexFunctor>
+    void crossZipIndexInPlace(DM<T2> &b, ZipIndexFunctor &f) {
+        this->updateDevice();
+        b.updateDevice();
+        if (this->n == 0) return;
+        detail::cross_zip_index_2d_kernel<T, T2, T, ZipIndexFunctor>
+            <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
+                this->d_data, this->d_data, b.getUserFunctionData(), (size_t)this->n, ncol, f);
+        detail::check_launch("DM::crossZipIndexInPlace");
+        this->deviceWritten();
+    }
+
+    // ---- row reductions and column writes (src/dm.cpp:1594-1871) ----------
+    // c[row] = fold of row `row` with f(acc, x): one wave per row, each lane
+    // folding its columns in order, then the 64 lane results in lane order.
+    // The row's own elements only (the reference GPU path; its CPU path
+    // starts from T(0)).
+    template <typename ReduceFunctor>
+    void reduceRows(DA<T> &c, ReduceFunctor &f) {
+        checkRowVector(c);
+        this->updateDevice();
+        c.updateDevice();
+        if (nrow == 0) return;
+        if (ncol == 0) throw detail::SizeMismatchException();
+        const int rows_per_block = detail::kThreads / 64;
+        detail::reduce_rows_kernel<T, ReduceFunctor>
+            <<<(nrow + rows_per_block - 1) / rows_per_block, detail::kThreads, 0, detail::stream()>>>(
+                c.getUserFunctionData(), this->d_data, nrow, ncol, f);
+        detail::check_launch("DM::reduceRows");
+        c.setCpuMemoryInSync(false);
+    }
+    // c[row] = f(m[row][ncol-1], m[row][0], ... f(m[row][0], m[row][1], T2(0)))
+    // in the reference's sequential order (src/dm.cpp:1731-1754).
+    template <typename T2, typename ReduceFunctor>
+    void reducetwoColumns(DA<T2> &c, ReduceFunctor &f) {
+        checkRowVector(c);
+        this->updateDevice();
+        c.updateDevice();
+        if (nrow == 0) return;
+        detail::reduce_two_columns_kernel<T, T2, ReduceFunctor>
+            <<<detail::grid_for(nrow), detail::kThreads, 0, detail::stream()>>>(c.getUserFunctionData(), this->d_data,
+                                                                                nrow, ncol, f);
+        detail::check_launch("DM::reducetwoColumns");
+        c.setCpuMemoryInSync(false);
+    }
+    // this[row][col] = da[row] (src/dm.cpp:1856-1880)
+    void setColumn(DA<T> &da, int col) {
+        if (col < 0 || col >= ncol) throw detail::IndexOutOfBoundsException();
+        if (da.getSize() != nrow) throw detail::SizeMismatchException();
+        this->updateDevice();
+        da.updateDevice();
+        if (nrow == 0) return;
+        detail::set_column_kernel<T><<<detail::grid_for(nrow), detail::kThreads, 0, detail::stream()>>>(
+            this->d_data, da.getUserFunctionData(), nrow, ncol, col);
+        detail::check_launch("DM::setColumn");
+        this->deviceWritten();
+    }
+    // one process holds every partition
+    void broadcastPartition(int /*partitionIndex*/) {}
+
     // ---- stencils (src/dm.cpp:1535-1591) ----------------------------------
     template <msl::NPLMMapStencilFunctor<T> f>
     void mapStencil(DM<T> &result, size_t stencilSize, T neutralValue, bool /*shared_mem*/ = true) {
@@ -204,17 +331,13 @@ public:
     int getnCol() { return ncol; }
     int getnRow() { return nrow; }
 
+    // one matrix row per line; like the reference (src/dm.cpp:271) it leaves
+    // std::cout in fixed notation
     void show(const std::string &descr = std::string(), int limited = 0) {
+        std::cout << std::fixed;
         this->updateHost();
-        std::ostringstream s;
-        if (!descr.empty()) s << descr << std::endl;
-        const int rows = (limited > 0 && limited < nrow) ? limited : nrow;
-        for (int i = 0; i < rows; i++) {
-            s << "[";
-            for (int j = 0; j < ncol; j++) s << this->localPartition[(size_t)i * ncol + j] << (j + 1 < ncol ? " " : "");
-            s << "]" << std::endl;
-        }
-        printv("%s", s.str().c_str());
+        const long count = (limited > 0 && limited < this->n) ? limited : this->n;
+        printv("%s", detail::show_text(descr, this->localPartition, count, ncol, 0).c_str());
     }
 
     int ncol;
@@ -234,6 +357,10 @@ private:
     template <typename U>
     void checkSameShape(const DM<U> &o) const {
         if (o.nrow != nrow || o.ncol != ncol) throw detail::NotSameSizeException();
+    }
+    template <typename U>
+    void checkRowVector(const DA<U> &v) const {
+        if (v.getSize() != nrow) throw detail::SizeMismatchException();
     }
     void rotate(int row_shift, int col_shift) {
         if (this->n == 0) return;

@@ -259,16 +259,15 @@ public:
 
     MSL_USERFUNC T *getUserFunctionData() { return d_data; }
 
-    void showLocal(const std::string &descr) { show(descr); }
+    void showLocal(const std::string &descr) {
+        updateHost();
+        printv("%s", detail::show_local_text(descr, localPartition, n).c_str());
+    }
+    // limited > 0 prints the first `limited` elements only
     void show(const std::string &descr = std::string(), int limited = 0) {
         updateHost();
-        std::ostringstream s;
-        if (!descr.empty()) s << descr << std::endl;
-        s << "[";
-        const int m = (limited > 0 && limited < n) ? limited : n;
-        for (int i = 0; i < m; i++) s << localPartition[i] << (i + 1 < m ? " " : "");
-        s << "]" << std::endl;
-        printv("%s", s.str().c_str());
+        const long count = (limited > 0 && limited < n) ? limited : n;
+        printv("%s", detail::show_text(descr, localPartition, count, 0, 0).c_str());
     }
     void printLocal() { show(); }
     long getnCPU() { return 0; }

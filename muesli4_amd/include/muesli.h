@@ -256,6 +256,45 @@ inline C2 proj2_2(C1, C2 b) {
     return b;
 }
 
+namespace detail {
+// Text of the containers' show(): "descr: " and a line break when descr is
+// given, then "[v0 v1 ... vlast]", a line break, and an empty line.  After
+// every row_len values the separator is "\n " instead of " ", and after
+// every plane_len values "\n-------------\n " (0 disables either) -- the
+// layout of src/ds.cpp:511-534, src/da.cpp:136-158, src/dm.cpp:266-296 and
+// src/dc.cpp:284-319, so that dumps of the two implementations diff.
+template <typename T>
+std::string show_text(const std::string &descr, const T *v, long count, long row_len, long plane_len) {
+    std::ostringstream s;
+    if (!descr.empty()) s << descr << ": " << std::endl;
+    s << "[";
+    for (long i = 0; i < count; i++) {
+        s << v[i];
+        if (i + 1 == count) break;
+        if (plane_len > 0 && (i + 1) % plane_len == 0)
+            s << "\n-------------\n ";
+        else if (row_len > 0 && (i + 1) % row_len == 0)
+            s << "\n ";
+        else
+            s << " ";
+    }
+    s << "]" << std::endl << std::endl;
+    return s.str();
+}
+
+// Text of showLocal(): "descr: [v0 v1 ... vlast ]" and a line break
+// (src/ds.cpp:493-508).
+template <typename T>
+std::string show_local_text(const std::string &descr, const T *v, long count) {
+    std::ostringstream s;
+    if (!descr.empty()) s << descr << ": ";
+    s << "[";
+    for (long i = 0; i < count; i++) s << v[i] << " ";
+    s << "]" << std::endl;
+    return s.str();
+}
+}  // namespace detail
+
 template <typename T>
 inline void show(T *a, int size) {
     std::ostringstream s;
@@ -263,6 +302,48 @@ inline void show(T *a, int size) {
     for (int i = 0; i < size; i++) s << a[i] << (i + 1 < size ? " " : "");
     s << "]" << std::endl;
     printv("%s", s.str().c_str());
+}
+
+// ---- per-thread random numbers for user functors (reference
+// include/muesli.h:609-641) ---------------------------------------------------
+// One counter-based SplitMix64 stream per (seed, index), the same on the
+// host and the device: generateRandomState(seed, i) starts the stream of
+// element i, randDouble draws uniformly from [min, max) and randInt from
+// [min, max].  (The reference draws from curand on the device and from the
+// unseeded host rand() on the CPU; neither sequence is reproduced here.)
+struct RandomState {
+    unsigned long long s;
+};
+typedef RandomState MSL_RANDOM_STATE;
+
+namespace detail {
+MSL_USERFUNC inline unsigned long long splitmix_next(RandomState &st) {
+    unsigned long long z = (st.s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+}  // namespace detail
+
+MSL_USERFUNC inline MSL_RANDOM_STATE generateRandomState(size_t seed, size_t someIndex) {
+    RandomState st{(unsigned long long)seed};
+    st.s ^= (unsigned long long)someIndex * 0xD1B54A32D192ED03ull;  // decorrelate neighbouring indices
+    detail::splitmix_next(st);
+    return st;
+}
+
+// uniform in [0, 1) with 53 random bits
+MSL_USERFUNC inline double randUnit(MSL_RANDOM_STATE &state) {
+    return (double)(detail::splitmix_next(state) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+MSL_USERFUNC inline double randDouble(double min, double max, MSL_RANDOM_STATE &state) {
+    return min + randUnit(state) * (max - min);
+}
+
+MSL_USERFUNC inline int randInt(int min, int max, MSL_RANDOM_STATE &state) {
+    const int v = min + (int)(randUnit(state) * ((double)max - (double)min + 1.0));
+    return v > max ? max : v;
 }
 
 }  // namespace msl

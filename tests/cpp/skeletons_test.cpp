@@ -107,6 +107,88 @@ public:
     MSL_USERFUNC int operator()(int x, int y, int z, int a, int b) const override { return (x + y + z) * a + b; }
 };
 
+// ---- functors of the extra DM / DA skeletons (the forms the ACO examples use)
+class Zip3Idx : public Functor5<int, int, int, int, int, int> {  // f(row, col, a, b, c)
+public:
+    MSL_USERFUNC int operator()(int row, int col, int a, int b, int c) const override {
+        return a + 2 * b + 3 * c + row * 1000 + col;
+    }
+};
+class RowOperand : public Functor4<int, int, int, double, int> {  // f(row, col, a, v[row])
+public:
+    MSL_USERFUNC int operator()(int row, int col, int a, double v) const override { return a + (int)v * 10 + col; }
+};
+class MatRowOperand : public Functor5<int, int, int, int, double, int> {  // f(row, col, a, b, v[row])
+public:
+    MSL_USERFUNC int operator()(int row, int col, int a, int b, double v) const override {
+        return a * b + (int)v + row;
+    }
+};
+class ThreeRowOperands : public Functor6<int, int, int, int, double, float, int> {  // f(row, col, a, u, v, w)
+public:
+    MSL_USERFUNC int operator()(int row, int col, int a, int u, double v, float w) const override {
+        return a + u * 100 + (int)v * 10 + (int)w + col;
+    }
+};
+class CrossTranspose {  // f(row, col, a, whole): reads the transposed element
+public:
+    explicit CrossTranspose(int n) : n(n) {}
+    MSL_USERFUNC int operator()(int row, int col, int a, int *whole) const { return a + whole[col * n + row]; }
+
+private:
+    int n;
+};
+class RowReduceSum : public Functor2<int, int, int> {
+public:
+    MSL_USERFUNC int operator()(int acc, int x) const override { return acc + x; }
+};
+class RowReduceMax : public Functor2<double, double, double> {
+public:
+    MSL_USERFUNC double operator()(double acc, double x) const override { return x > acc ? x : acc; }
+};
+class PairProduct : public Functor3<int, int, long long, long long> {  // f(x, next, acc)
+public:
+    MSL_USERFUNC long long operator()(int x, int nx, long long acc) const override {
+        return acc * 3 + (long long)x * nx;
+    }
+};
+class RowEnds {  // DA: f(i, &rows[i][0], v)
+public:
+    explicit RowEnds(int cols) : cols(cols) {}
+    MSL_USERFUNC int operator()(int i, int *row, int v) const { return v + row[0] + row[cols - 1] + i; }
+
+private:
+    int cols;
+};
+class RowDot {  // DA: f(i, &rows[i][0], b[i], v)
+public:
+    explicit RowDot(int cols) : cols(cols) {}
+    MSL_USERFUNC int operator()(int i, int *row, int b, int v) const {
+        int s = 0;
+        for (int c = 0; c < cols; c++) s += row[c] * (c + 1);
+        return s * b + v - i;
+    }
+
+private:
+    int cols;
+};
+class DrawInts {  // per-element random draws: f(i, v)
+public:
+    MSL_USERFUNC int operator()(int i, int) const {
+        MSL_RANDOM_STATE st = generateRandomState(1234, i);
+        int acc = 0;
+        for (int k = 0; k < 5; k++) acc = acc * 31 + randInt(3, 17, st);
+        return acc;
+    }
+};
+class DrawDoubles {
+public:
+    MSL_USERFUNC double operator()(int i, double) const {
+        MSL_RANDOM_STATE st = generateRandomState(99, i);
+        return randDouble(-2.0, 5.0, st);
+    }
+};
+
 // Game of Life in the legacy mapStencilMM form (examples/gameoflife.cpp:18-43).
 class LifeMM : public DMMapStencilFunctor<int, int> {
 public:
@@ -307,6 +389,248 @@ static void closed_forms() {
     check(empty.getSize() == 0, "DA empty");
 }
 
+// ---- the extra DM / DA skeletons and the random-number API -------------------
+static void extra_skeletons() {
+    const int rows = 23, cols = 67;
+    DM<int> a(rows, cols), b(rows, cols), c(rows, cols);
+    std::vector<int> ha(rows * cols), hb(rows * cols), hc(rows * cols);
+    for (int k = 0; k < rows * cols; k++) {
+        ha[k] = (k * 7) % 13 - 6;
+        hb[k] = (k * 5) % 11;
+        hc[k] = k % 9;
+    }
+    a.fill(ha.data());
+    b.fill(hb.data());
+    c.fill(hc.data());
+    DA<double> vd(rows);
+    DA<int> vi(rows);
+    DA<float> vf(rows);
+    for (int r = 0; r < rows; r++) {
+        vd.set(r, r * 0.5 + 1.0);
+        vi.set(r, r - 4);
+        vf.set(r, (float)(r % 5));
+    }
+
+    Zip3Idx z3;
+    a.zipIndexInPlace3(b, c, z3);
+    bool ok = true;
+    for (int r = 0; r < rows; r++)
+        for (int q = 0; q < cols; q++) {
+            const int k = r * cols + q;
+            ok &= a.get2D(r, q) == ha[k] + 2 * hb[k] + 3 * hc[k] + r * 1000 + q;
+            ha[k] = ha[k] + 2 * hb[k] + 3 * hc[k] + r * 1000 + q;
+        }
+    check(ok, "DM zipIndexInPlace3");
+
+    MatRowOperand mro;
+    a.zipIndexInPlaceMA(b, vd, mro);
+    ok = true;
+    for (int r = 0; r < rows; r++)
+        for (int q = 0; q < cols; q++) {
+            const int k = r * cols + q;
+            ha[k] = ha[k] * hb[k] + (int)(r * 0.5 + 1.0) + r;
+            ok &= a.get2D(r, q) == ha[k];
+        }
+    check(ok, "DM zipIndexInPlaceMA");
+
+    RowOperand ro;
+    a.zipIndexInPlaceA(vd, ro);
+    ok = true;
+    for (int r = 0; r < rows; r++)
+        for (int q = 0; q < cols; q++) {
+            const int k = r * cols + q;
+            ha[k] = ha[k] + (int)(r * 0.5 + 1.0) * 10 + q;
+            ok &= a.get2D(r, q) == ha[k];
+        }
+    check(ok, "DM zipIndexInPlaceA");
+
+    ThreeRowOperands tro;
+    a.zipIndexInPlaceAAA(vi, vd, vf, tro);
+    ok = true;
+    for (int r = 0; r < rows; r++)
+        for (int q = 0; q < cols; q++) {
+            const int k = r * cols + q;
+            ha[k] = ha[k] + (r - 4) * 100 + (int)(r * 0.5 + 1.0) * 10 + (r % 5) + q;
+            ok &= a.get2D(r, q) == ha[k];
+        }
+    check(ok, "DM zipIndexInPlaceAAA");
+
+    bool threw = false;
+    try {
+        DA<double> shortv(rows - 1);
+        a.zipIndexInPlaceA(shortv, ro);
+    } catch (const detail::SizeMismatchException &) {
+        threw = true;
+    }
+    check(threw, "DM zipIndexInPlaceA size mismatch");
+
+    // crossZipIndexInPlace on a square matrix: this += transpose(b)
+    const int sq = 40;
+    DM<int> s1(sq, sq), s2(sq, sq);
+    std::vector<int> h1(sq * sq), h2(sq * sq);
+    for (int k = 0; k < sq * sq; k++) {
+        h1[k] = k;
+        h2[k] = 3 * k + 1;
+    }
+    s1.fill(h1.data());
+    s2.fill(h2.data());
+    CrossTranspose ct(sq);
+    s1.crossZipIndexInPlace(s2, ct);
+    ok = true;
+    for (int r = 0; r < sq; r++)
+        for (int q = 0; q < sq; q++) ok &= s1.get2D(r, q) == h1[r * sq + q] + h2[q * sq + r];
+    check(ok, "DM crossZipIndexInPlace");
+
+    // reduceRows: exact int sums and double max over rows of 1, 63, 64, 65 and
+    // 300 columns (fewer, equal and more columns than a wave's lanes)
+    for (int rc : {1, 63, 64, 65, 300}) {
+        const int rr = 37;
+        DM<int> m(rr, rc);
+        DM<double> md(rr, rc);
+        std::vector<int> hm(rr * rc);
+        std::vector<double> hd(rr * rc);
+        for (int k = 0; k < rr * rc; k++) {
+            hm[k] = (k * 13) % 101 - 50;
+            hd[k] = -1000.0 + ((k * 37) % 997) * 0.25;
+        }
+        m.fill(hm.data());
+        md.fill(hd.data());
+        DA<int> sums(rr);
+        DA<double> maxes(rr);
+        RowReduceSum rs;
+        RowReduceMax rm;
+        m.reduceRows(sums, rs);
+        md.reduceRows(maxes, rm);
+        ok = true;
+        for (int r = 0; r < rr; r++) {
+            int want = 0;
+            double wmax = hd[r * rc];
+            for (int q = 0; q < rc; q++) {
+                want += hm[r * rc + q];
+                wmax = hd[r * rc + q] > wmax ? hd[r * rc + q] : wmax;
+            }
+            ok &= sums.get(r) == want && maxes.get(r) == wmax;
+        }
+        check(ok, "DM reduceRows cols=" + std::to_string(rc));
+    }
+
+    // reducetwoColumns: sequential f(x, next, acc) from 0, wrapping to column 0
+    {
+        const int rr = 9, rc = 6;
+        DM<int> m(rr, rc);
+        std::vector<int> hm(rr * rc);
+        for (int k = 0; k < rr * rc; k++) hm[k] = (k % 7) - 3;
+        m.fill(hm.data());
+        DA<long long> out(rr);
+        PairProduct pp;
+        m.reducetwoColumns(out, pp);
+        ok = true;
+        for (int r = 0; r < rr; r++) {
+            long long acc = 0;
+            for (int q = 0; q < rc; q++) acc = acc * 3 + (long long)hm[r * rc + q] * hm[r * rc + (q + 1) % rc];
+            ok &= out.get(r) == acc;
+        }
+        check(ok, "DM reducetwoColumns");
+    }
+
+    // setColumn and its argument checks
+    {
+        DM<int> m(rows, cols, 7);
+        DA<int> col(rows);
+        for (int r = 0; r < rows; r++) col.set(r, 100 + r);
+        m.setColumn(col, 5);
+        ok = true;
+        for (int r = 0; r < rows; r++)
+            for (int q = 0; q < cols; q++) ok &= m.get2D(r, q) == (q == 5 ? 100 + r : 7);
+        check(ok, "DM setColumn");
+        bool bad_col = false, bad_len = false;
+        try {
+            m.setColumn(col, cols);
+        } catch (const detail::IndexOutOfBoundsException &) {
+            bad_col = true;
+        }
+        try {
+            DA<int> shortc(rows - 2);
+            m.setColumn(shortc, 0);
+        } catch (const detail::SizeMismatchException &) {
+            bad_len = true;
+        }
+        check(bad_col && bad_len, "DM setColumn argument checks");
+    }
+
+    // DA with matrix rows
+    {
+        const int na = 30, mc = 11;
+        DM<int> m(na + 3, mc);  // more rows than array elements is allowed
+        std::vector<int> hm((na + 3) * mc);
+        for (int k = 0; k < (na + 3) * mc; k++) hm[k] = (k * 3) % 17;
+        m.fill(hm.data());
+        DA<int> x(na, 5), y(na);
+        for (int i = 0; i < na; i++) y.set(i, i % 4);
+        RowEnds re(mc);
+        x.mapIndexInPlaceDMRows(m, re);
+        ok = true;
+        std::vector<int> hx(na);
+        for (int i = 0; i < na; i++) {
+            hx[i] = 5 + hm[i * mc] + hm[i * mc + mc - 1] + i;
+            ok &= x.get(i) == hx[i];
+        }
+        check(ok, "DA mapIndexInPlaceDMRows");
+        RowDot rd(mc);
+        x.zipIndexInPlaceDMRows(y, m, rd);
+        ok = true;
+        for (int i = 0; i < na; i++) {
+            int sdot = 0;
+            for (int q = 0; q < mc; q++) sdot += hm[i * mc + q] * (q + 1);
+            ok &= x.get(i) == sdot * (i % 4) + hx[i] - i;
+        }
+        check(ok, "DA zipIndexInPlaceDMRows");
+    }
+
+    // random numbers: the device draws equal the host draws, and stay in range
+    {
+        const int nr = 5000;
+        DA<int> ri(nr, 0);
+        DA<double> rdv(nr, 0.0);
+        DrawInts di;
+        DrawDoubles dd;
+        ri.mapIndexInPlace(di);
+        rdv.mapIndexInPlace(dd);
+        ok = true;
+        bool in_range = true;
+        int lo = 100, hi = -1;
+        for (int i = 0; i < nr; i++) {
+            ok &= ri.get(i) == di(i, 0) && rdv.get(i) == dd(i, 0.0);
+            MSL_RANDOM_STATE st = generateRandomState(1234, i);
+            for (int k = 0; k < 5; k++) {
+                const int v = randInt(3, 17, st);
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+            in_range &= rdv.get(i) >= -2.0 && rdv.get(i) < 5.0;
+        }
+        check(ok, "random numbers: device == host");
+        check(in_range && lo == 3 && hi == 17, "random numbers: ranges");
+    }
+
+    // show() layouts (compared with the reference's text by the Python side)
+    {
+        DA<int> sa(5);
+        DM<int> sm(2, 3);
+        DC<int> sc(2, 2, 2);
+        for (int i = 0; i < 5; i++) sa.set(i, i * 10);
+        for (int i = 0; i < 6; i++) sm.set(i, i + 1);
+        for (int i = 0; i < 8; i++) sc.set(i, i);
+        fflush(stdout);
+        sa.show("sa");
+        sa.show("", 3);
+        sa.showLocal("sal");
+        sm.show("sm");
+        sc.show("sc");
+        fflush(stdout);
+    }
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) {
         fprintf(stderr, "usage: %s <dir>\n", argv[0]);
@@ -316,6 +640,7 @@ int main(int argc, char **argv) {
     msl::initSkeletons(argc, argv);
     try {
         closed_forms();
+        extra_skeletons();
 
         int d[3];
         // ---- Game of Life: legacy functor, function pointer, library kernel

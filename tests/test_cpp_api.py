@@ -22,6 +22,8 @@ HOT_PATH_EXAMPLES = [
     "heatdiffusion", "gameoflife", "jacobisolver", "gaussianblur", "piDA",
     "meanblur2", "meanblur4", "meanblur6", "meanblur8", "meanblur10", "meanblur12",
     "meanblur14", "meanblur16", "meanblur18", "meanblur20",
+    # the other examples the reference's own build compiles (CMakeLists.txt:301-317)
+    "aco_tsp", "aco_bpp", "gaussianblur-color", "minnested",
 ]
 
 needs_hipcc = pytest.mark.skipif(not os.path.exists(B.HIPCC) and shutil.which("hipcc") is None,
@@ -30,7 +32,8 @@ needs_hipcc = pytest.mark.skipif(not os.path.exists(B.HIPCC) and shutil.which("h
 
 # stencil-heavy examples get a full device compile (code generation can
 # fail where type-checking passes); the rest are type-checked
-DEVICE_COMPILED = ["gameoflife", "heatdiffusion", "jacobisolver", "gaussianblur", "meanblur2", "dc_map_stencil_test"]
+DEVICE_COMPILED = ["gameoflife", "heatdiffusion", "jacobisolver", "gaussianblur", "meanblur2", "dc_map_stencil_test",
+                   "aco_tsp", "aco_bpp"]
 
 
 def _syntax(src, codegen=False):

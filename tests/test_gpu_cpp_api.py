@@ -142,3 +142,27 @@ def test_user_fold_float(run):
     got = float(_read(run["dir"] / "fold_sum.bin", np.float32).ravel()[0])
     ref = math.fsum(run["blur"].astype(np.float64).ravel())
     assert abs(got - ref) <= FOLD_REL_TOL * abs(ref), (got, ref)
+
+
+def test_extra_skeletons_and_random_numbers(run):
+    """zipIndexInPlace3/MA/A/AAA, crossZipIndexInPlace, reduceRows,
+    reducetwoColumns, setColumn, DA::*DMRows and the random-number API,
+    checked in-process against host loops (exact integers; device draws
+    equal host draws)."""
+    names = ["DM zipIndexInPlace3", "DM zipIndexInPlaceMA", "DM zipIndexInPlaceA", "DM zipIndexInPlaceAAA",
+             "DM crossZipIndexInPlace", "DM reduceRows cols=1", "DM reduceRows cols=65", "DM reduceRows cols=300",
+             "DM reducetwoColumns", "DM setColumn", "DM setColumn argument checks", "DA mapIndexInPlaceDMRows",
+             "DA zipIndexInPlaceDMRows", "random numbers: device == host", "random numbers: ranges"]
+    for name in names:
+        assert f"OK   {name}\n" in run["log"], name
+
+
+def test_show_layouts(run):
+    """show()/showLocal() print the reference's text layout
+    (src/ds.cpp:493-534, src/dm.cpp:266-296, src/dc.cpp:284-319)."""
+    want = ("sa: \n[0 10 20 30 40]\n\n"
+            "[0 10 20]\n\n"
+            "sal: [0 10 20 30 40 ]\n"
+            "sm: \n[1 2 3\n 4 5 6]\n\n"
+            "sc: \n[0 1\n 2 3\n-------------\n 4 5\n 6 7]\n\n")
+    assert want in run["log"], run["log"][-800:]
