@@ -46,7 +46,11 @@ CODEGEN_FLAGS = [
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_FLAGS = [*CODEGEN_FLAGS, "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+# The strip marches fully unroll one row-sum ring (up to 33 rows at r = 16)
+# so that every ring slot is a register; the default size limit of
+# `#pragma unroll` gives up on the widest ones and falls back to scratch.
+HIP_FLAGS = [*CODEGEN_FLAGS, "-mllvm", "-pragma-unroll-threshold=200000", "-Wall", "-Wno-unused-function",
+             "-Wno-unused-variable"]
 
 
 def _run(cmd, cwd=None):

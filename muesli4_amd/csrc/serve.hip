@@ -79,9 +79,13 @@ struct SFinal {
     int kind, item0, nitems, slot;
 };
 
-// item_desc[i] = largest j with starts[j] <= i (starts ascending, starts[0] = 0)
-__global__ void serve_expand_kernel(const int *__restrict__ starts, int ndesc, int nitems, int *__restrict__ item_desc) {
+// item_desc[i] = largest j with starts[j] <= i (starts ascending, starts[0] = 0);
+// also clears the round's work-queue counter of this kind (the work kernel
+// runs after this one on the same stream).
+__global__ void serve_expand_kernel(const int *__restrict__ starts, int ndesc, int nitems, int *__restrict__ item_desc,
+                                    unsigned *__restrict__ ctr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *ctr = 0u;
     if (i >= nitems) return;
     int lo = 0, hi = ndesc - 1;
     while (lo < hi) {
@@ -92,14 +96,23 @@ __global__ void serve_expand_kernel(const int *__restrict__ starts, int ndesc, i
     item_desc[i] = lo;
 }
 
+// Each wave takes its first item statically and every further one from the
+// round's atomic counter (items differ in cost by radius and shape, so a
+// static split leaves a long tail).
+__device__ __forceinline__ int serve_next_item(unsigned *ctr, int stride) {
+    unsigned nx = 0;
+    if ((threadIdx.x & 63) == 0) nx = atomicAdd(ctr, 1u);
+    return stride + __builtin_amdgcn_readfirstlane((int)nx);
+}
+
 __global__ __launch_bounds__(256) void serve_blur_kernel(const SBlur *__restrict__ descs,
                                                          const int *__restrict__ item_desc, int nitems,
-                                                         float2 *__restrict__ partials) {
+                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
     __shared__ __attribute__((aligned(16))) float lds[4][2 * kBatchLdsw];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nw = gridDim.x * 4;
     for (int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave); i < nitems;
-         i += nw) {
+         i = serve_next_item(ctr, nw)) {
         const int j = __builtin_amdgcn_readfirstlane(item_desc[i]);
         const SBlur g = descs[j];
         const int k = i - g.item0;
@@ -121,11 +134,11 @@ __global__ __launch_bounds__(256) void serve_blur_kernel(const SBlur *__restrict
 
 __global__ __launch_bounds__(256) void serve_heat_kernel(const SHeat *__restrict__ descs,
                                                          const int *__restrict__ item_desc, int nitems,
-                                                         float2 *__restrict__ partials) {
+                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nw = gridDim.x * 4;
     for (int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave); i < nitems;
-         i += nw) {
+         i = serve_next_item(ctr, nw)) {
         const int j = __builtin_amdgcn_readfirstlane(item_desc[i]);
         const SHeat g = descs[j];
         const int k = i - g.item0;
@@ -296,6 +309,7 @@ struct RoundSlot {
     int *d_item_desc[3] = {nullptr, nullptr, nullptr};
     float2 *d_pb = nullptr, *d_ph = nullptr;
     uint32_t *d_pm = nullptr;
+    unsigned *d_ctr = nullptr;  // work-queue counters: blur, heat, map
     double *d_rf = nullptr, *h_rf = nullptr;
     uint32_t *d_ru = nullptr, *h_ru = nullptr;
     std::vector<std::pair<SJob *, int>> finishing;
@@ -388,6 +402,7 @@ static void slot_free(RoundSlot &R) {
     if (R.d_pb) (void)hipFree(R.d_pb);
     if (R.d_ph) (void)hipFree(R.d_ph);
     if (R.d_pm) (void)hipFree(R.d_pm);
+    if (R.d_ctr) (void)hipFree(R.d_ctr);
     if (R.d_rf) (void)hipFree(R.d_rf);
     if (R.d_ru) (void)hipFree(R.d_ru);
     if (R.h_rf) (void)hipHostFree(R.h_rf);
@@ -397,6 +412,7 @@ static void slot_free(RoundSlot &R) {
 
 // Grows a slot's buffers (only called while the slot is not in flight).
 static int slot_reserve(RoundSlot &R, size_t stage_bytes, const int items[3]) {
+    if (!R.d_ctr) MSL_HIP_TRY(hipMalloc(&R.d_ctr, sizeof(unsigned) * 3));
     if (stage_bytes > R.stage_cap) {
         if (R.h_stage) MSL_HIP_TRY(hipHostFree(R.h_stage));
         if (R.d_stage) MSL_HIP_TRY(hipFree(R.d_stage));
@@ -630,15 +646,19 @@ static int run_round(Server &S, int *did_work) {
     const int cus = ctx().num_cus;
     auto persistent = [&](int n, int per_cu) { return std::max(1, std::min((n + 3) / 4, cus * per_cu)); };
     if (items[0]) {
-        serve_expand_kernel<<<(items[0] + 255) / 256, 256, 0, st>>>(gsb, (int)db.size(), items[0], R.d_item_desc[0]);
-        serve_blur_kernel<<<persistent(items[0], 3), 256, 0, st>>>(gb, R.d_item_desc[0], items[0], R.d_pb);
+        serve_expand_kernel<<<(items[0] + 255) / 256, 256, 0, st>>>(gsb, (int)db.size(), items[0], R.d_item_desc[0],
+                                                                    R.d_ctr);
+        serve_blur_kernel<<<persistent(items[0], 3), 256, 0, st>>>(gb, R.d_item_desc[0], items[0], R.d_pb, R.d_ctr);
     }
     if (items[1]) {
-        serve_expand_kernel<<<(items[1] + 255) / 256, 256, 0, st>>>(gsh, (int)dh.size(), items[1], R.d_item_desc[1]);
-        serve_heat_kernel<<<persistent(items[1], 3), 256, 0, st>>>(gh, R.d_item_desc[1], items[1], R.d_ph);
+        serve_expand_kernel<<<(items[1] + 255) / 256, 256, 0, st>>>(gsh, (int)dh.size(), items[1], R.d_item_desc[1],
+                                                                    R.d_ctr + 1);
+        serve_heat_kernel<<<persistent(items[1], 3), 256, 0, st>>>(gh, R.d_item_desc[1], items[1], R.d_ph,
+                                                                   R.d_ctr + 1);
     }
     if (items[2]) {
-        serve_expand_kernel<<<(items[2] + 255) / 256, 256, 0, st>>>(gsm, (int)dm.size(), items[2], R.d_item_desc[2]);
+        serve_expand_kernel<<<(items[2] + 255) / 256, 256, 0, st>>>(gsm, (int)dm.size(), items[2], R.d_item_desc[2],
+                                                                    R.d_ctr + 2);
         serve_map_kernel<<<persistent(items[2], 4), 256, 0, st>>>(gm, R.d_item_desc[2], items[2], R.d_pm);
     }
     if (!fin.empty()) {

@@ -42,9 +42,9 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     const int sy = item / strips_x;
     const int y0 = sy * seg;
     // Grids that stay in the Infinity Cache run fewer waves per CU (see
-    // strip_shape), so radii that otherwise prefetch one row ahead prefetch
-    // three there (measured: r = 5, 6 and 8 gain 4-25 % on 4096^2).
-    constexpr int DQ = (NT || StripConsts<R, C>::D > 1) ? 0 : 3;
+    // strip_shape), so radii up to 10 that otherwise prefetch one row ahead
+    // prefetch three there (measured: r = 5, 6 and 8 gain 4-25 % on 4096^2).
+    constexpr int DQ = (NT || StripConsts<R, C>::D > 1 || R > 10) ? 0 : 3;
     strip_march<Pol, C, MODE, false, DQ, NT>(out, in, rows, cols, nv, sx * StripConsts<R, C>::SW, y0,
                                              min(y0 + seg, rows), pol, lds[wave], nullptr, sy & 1);
 }

@@ -274,7 +274,7 @@ def test_gol_1024_100_iterations_and_fold():
 WIDE_SHAPES_2D = [(1900, 2050), (16400, 3), (64, 65536), (2048, 2048)]
 
 
-@pytest.mark.parametrize("r", [1, 3, 7, 10])
+@pytest.mark.parametrize("r", [1, 3, 7, 10, 12, 16])
 def test_mean_blur_wide_strip_modes(r):
     for (rows, cols) in WIDE_SHAPES_2D:
         for nv in (0.0, 3.5):
