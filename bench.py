@@ -121,6 +121,18 @@ def main():
     log(f"[bench] kernels {kernels}")
     flush.free()
 
+    # ---- fold error of the headline grid: the compensated FP32 fold (FP64
+    # final combine) against an FP64 (pairwise) sum of the same floats
+    fresh()
+    s64, s32 = C.c_double(), C.c_float()
+    call("msl_fold_sum_f32_host", a.ptr, n, C.byref(s64), C.byref(s32))
+    host = a.to_host().ravel()
+    ref64 = float(np.sum(host, dtype=np.float64))
+    fold_check = {"fold_sum_f64_combine": s64.value, "fold_sum_f32": float(s32.value), "fp64_reference": ref64,
+                  "rel_err": abs(s64.value - ref64) / abs(ref64), "tolerance": 1e-6}
+    del host
+    log(f"[bench] fold check {fold_check}")
+
     # ---- headline step ------------------------------------------------------
     def step():
         call("msl_map_affine_f32", c.ptr, a.ptr, n, 1.5, 3.0)
@@ -196,6 +208,7 @@ def main():
         "roofline": roofline,
         "per_radius": per_radius,
         "kernels": kernels,
+        "fold_check": fold_check,
         "kernels_note": "each launch timed alone (HIP events on the library stream) after a 1 GiB read that "
                         "evicts the 256 MiB Infinity Cache; blur kernels write a third grid",
         "device": info,

@@ -81,10 +81,11 @@ public:
         const int r = (int)stencilSize;
         const size_t lds = detail::march3d_lds_bytes(r, sizeof(T));
         if (lds <= detail::kMaxLdsBytes) {
-            const detail::March3D m = detail::march3d_shape(ncol, nrow, depth);
-            const int mask = detail::march3d_ring_planes(r) - 1;
-            detail::dc_stencil_kernel<T, f><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
-                result.d_data, this->d_data, ncol, nrow, depth, r, neutralValue, m.xtiles, m.ytiles, m.zseg, mask);
+            switch (r) {  // compile-time radius for the common small ones
+                case 1: marchStencil<f, 1>(result, r, neutralValue, lds); break;
+                case 2: marchStencil<f, 2>(result, r, neutralValue, lds); break;
+                default: marchStencil<f, 0>(result, r, neutralValue, lds); break;
+            }
         } else {
             detail::dc_stencil_global_kernel<T, f><<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
                 result.d_data, this->d_data, ncol, nrow, depth, r, neutralValue);
@@ -134,6 +135,13 @@ public:
     bool rowComplete = true;
 
 private:
+    template <msl::DCMapStencilFunctor<T> f, int RC>
+    void marchStencil(DC<T> &result, int r, T nv, size_t lds) {
+        const detail::March3D m = detail::march3d_shape(ncol, nrow, depth);
+        const int mask = detail::march3d_ring_planes(r) - 1;
+        detail::dc_stencil_kernel<T, f, RC><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+            result.d_data, this->d_data, ncol, nrow, depth, r, nv, m.xtiles, m.ytiles, m.zseg, mask);
+    }
     template <typename In, typename F>
     void launch_index_map(const In *in, F &f) {
         if (this->n == 0) return;

@@ -368,13 +368,18 @@ constexpr int kMarchW = 256;
 constexpr int kMarchK = 4;
 constexpr size_t kMaxLdsBytes = 48 * 1024;
 
-inline int pow2_at_least(int v) {
+constexpr int pow2_at_least(int v) {
     int h = 1;
     while (h < v) h <<= 1;
     return h;
 }
 
-inline int march2d_ring_rows(int r) { return pow2_at_least(2 * r + 2 * kMarchK); }
+constexpr int march2d_ring_rows(int r) { return pow2_at_least(2 * r + 2 * kMarchK); }
+
+// The generic stencil kernels take the radius as a template argument RC for
+// the common small radii (RC = 0: at run time).  With a constant radius the
+// view's window geometry is constant too, and the compiler folds the
+// per-access window checks of constant-offset reads.
 
 inline size_t march2d_lds_bytes(int r, size_t elem) {
     return (size_t)march2d_ring_rows(r) * (size_t)(kMarchW + 2 * r) * elem;
@@ -442,10 +447,13 @@ __device__ __forceinline__ void march2d(const T *__restrict__ in, int rows, int 
     }
 }
 
-template <typename T, T (*f)(const NPLMatrix<T> &, int, int)>
-__global__ __launch_bounds__(kThreads) void dm_stencil_kernel(T *out, const T *in, int rows, int cols, int r, T nv,
-                                                              int strips_x, int seg, int mask) {
+template <typename T, T (*f)(const NPLMatrix<T> &, int, int), int RC = 0>
+__global__ __launch_bounds__(kThreads) void dm_stencil_kernel(T *out, const T *in, int rows, int cols, int r_arg, T nv,
+                                                              int strips_x, int seg, int mask_arg) {
+    const int r = RC > 0 ? RC : r_arg;
+    const int mask = RC > 0 ? march2d_ring_rows(RC) - 1 : mask_arg;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    __builtin_assume(__builtin_amdgcn_workitem_id_x() < (unsigned)kThreads);  // lets window checks fold
     T *ring = reinterpret_cast<T *>(smem_raw);
     const int xs = (blockIdx.x % strips_x) * kMarchW;
     const int y0 = (blockIdx.x / strips_x) * seg;
@@ -491,11 +499,14 @@ __global__ __launch_bounds__(kThreads) void dm_stencil_global_kernel(T *out, con
 }
 
 // Legacy mapStencilMM: result(row, col) = f(row, col, &view, ncol, nrow).
-template <typename T, typename R, typename F>
-__global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T *in, int rows, int cols, int r, T nv,
-                                                                 const T *border_data, int border, int strips_x,
-                                                                 int seg, int mask, F f) {
+template <typename T, typename R, typename F, int RC = 0>
+__global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T *in, int rows, int cols, int r_arg,
+                                                                 T nv, const T *border_data, int border, int strips_x,
+                                                                 int seg, int mask_arg, F f) {
+    const int r = RC > 0 ? RC : r_arg;
+    const int mask = RC > 0 ? march2d_ring_rows(RC) - 1 : mask_arg;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    __builtin_assume(__builtin_amdgcn_workitem_id_x() < (unsigned)kThreads);  // lets window checks fold
     T *ring = reinterpret_cast<T *>(smem_raw);
     const int xs = (blockIdx.x % strips_x) * kMarchW;
     const int y0 = (blockIdx.x / strips_x) * seg;
@@ -580,7 +591,7 @@ constexpr int kM3Y = 8;
 constexpr int kMarch3K = 2;
 constexpr int kM3Stage = 4;  // staged cells per thread per plane (r <= 3)
 
-inline int march3d_ring_planes(int r) { return pow2_at_least(2 * r + 2 * kMarch3K); }
+constexpr int march3d_ring_planes(int r) { return pow2_at_least(2 * r + 2 * kMarch3K); }
 
 inline size_t march3d_lds_bytes(int r, size_t elem) {
     const size_t plane = (size_t)(kM3X + 2 * r) * (kM3Y + 2 * r);
@@ -607,10 +618,13 @@ inline March3D march3d_shape(int cols, int rows, int depth) {
     return m;
 }
 
-template <typename T, T (*f)(const PLCube<T> &, int, int, int)>
+template <typename T, T (*f)(const PLCube<T> &, int, int, int), int RC = 0>
 __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *in, int cols, int rows, int depth,
-                                                              int r, T nv, int xtiles, int ytiles, int zseg,
-                                                              int mask) {
+                                                              int r_arg, T nv, int xtiles, int ytiles, int zseg,
+                                                              int mask_arg) {
+    const int r = RC > 0 ? RC : r_arg;
+    const int mask = RC > 0 ? march3d_ring_planes(RC) - 1 : mask_arg;
+    __builtin_assume(__builtin_amdgcn_workitem_id_x() < (unsigned)kThreads);  // lets window checks fold
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *ring = reinterpret_cast<T *>(smem_raw);
     const int bx = blockIdx.x % xtiles;

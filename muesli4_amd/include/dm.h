@@ -228,10 +228,13 @@ public:
         const int r = (int)stencilSize;
         const size_t lds = detail::march2d_lds_bytes(r, sizeof(T));
         if (lds <= detail::kMaxLdsBytes) {
-            const detail::March2D m = detail::march2d_shape(nrow, ncol);
-            const int mask = detail::march2d_ring_rows(r) - 1;
-            detail::dm_stencil_kernel<T, f><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
-                result.d_data, this->d_data, nrow, ncol, r, neutralValue, m.strips_x, m.seg, mask);
+            switch (r) {  // compile-time radius for the common small ones
+                case 1: marchStencil<f, 1>(result, r, neutralValue, lds); break;
+                case 2: marchStencil<f, 2>(result, r, neutralValue, lds); break;
+                case 3: marchStencil<f, 3>(result, r, neutralValue, lds); break;
+                case 4: marchStencil<f, 4>(result, r, neutralValue, lds); break;
+                default: marchStencil<f, 0>(result, r, neutralValue, lds); break;
+            }
         } else {
             detail::dm_stencil_global_kernel<T, f><<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
                 result.d_data, this->d_data, nrow, ncol, r, neutralValue);
@@ -383,11 +386,11 @@ private:
         const int r = f.getStencilSize();
         const size_t lds = detail::march2d_lds_bytes(r, sizeof(T));
         if (lds <= detail::kMaxLdsBytes) {
-            const detail::March2D m = detail::march2d_shape(nrow, ncol);
-            const int mask = detail::march2d_ring_rows(r) - 1;
-            detail::dm_stencil_mm_kernel<T, T2, F><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
-                result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, m.strips_x, m.seg, mask,
-                f);
+            switch (r) {  // compile-time radius for the common small ones
+                case 1: marchStencilMM<1>(result, f, r, nv, border, border_w, lds); break;
+                case 2: marchStencilMM<2>(result, f, r, nv, border, border_w, lds); break;
+                default: marchStencilMM<0>(result, f, r, nv, border, border_w, lds); break;
+            }
         } else {
             detail::dm_stencil_mm_global_kernel<T, T2, F>
                 <<<detail::grid_for(this->n), detail::kThreads, 0, detail::stream()>>>(
@@ -395,6 +398,21 @@ private:
         }
         detail::check_launch("DM::mapStencilMM");
         result.setCpuMemoryInSync(false);
+    }
+    template <msl::NPLMMapStencilFunctor<T> f, int RC>
+    void marchStencil(DM<T> &result, int r, T nv, size_t lds) {
+        const detail::March2D m = detail::march2d_shape(nrow, ncol);
+        const int mask = detail::march2d_ring_rows(r) - 1;
+        detail::dm_stencil_kernel<T, f, RC><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+            result.d_data, this->d_data, nrow, ncol, r, nv, m.strips_x, m.seg, mask);
+    }
+    template <int RC, typename T2, typename F>
+    void marchStencilMM(DM<T2> &result, F &f, int r, T nv, const T *border, int border_w, size_t lds) {
+        const detail::March2D m = detail::march2d_shape(nrow, ncol);
+        const int mask = detail::march2d_ring_rows(r) - 1;
+        detail::dm_stencil_mm_kernel<T, T2, F, RC><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+            result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, m.strips_x, m.seg, mask,
+            f);
     }
     template <typename In, typename F>
     void launch_index_map(const In *in, F &f) {
