@@ -20,20 +20,11 @@ constexpr int kTX = 64;  // x extent of a wave tile (one cell per lane)
 constexpr int kTY = 8;   // y extent of a wave tile (cells per lane)
 constexpr int kP = 4;    // plane ring: z-1, z, z+1 and the requested z+2
 
-// Branch-free cell read: the address is clamped into the cube so the load is
-// never predicated; cell_ok tells afterwards whether the cell was inside (its
-// value is replaced by nv when it was not).  The march applies the fix-up
-// when a plane is first used, not when it is requested, so requested planes
-// stay in flight.
-__device__ __forceinline__ float ld_cell(const float *__restrict__ in, int x, int y, int z, int nx, int ny, int nz) {
-    const int xc = min(max(x, 0), nx - 1);
-    const int yc = min(max(y, 0), ny - 1);
-    const int zc = min(max(z, 0), nz - 1);
-    return in[((size_t)zc * ny + yc) * nx + xc];
-}
-__device__ __forceinline__ bool cell_ok(int x, int y, int z, int nx, int ny, int nz) {
-    return (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && (unsigned)z < (unsigned)nz;
-}
+// Reads are branch-free: addresses are clamped into the cube so no load is
+// predicated, and the neutral value replaces the cells outside it when a
+// plane is first used, not when it is requested, so requested planes stay
+// in flight.  In-plane byte offsets are 32-bit (planes of fewer than 2^30
+// cells).
 
 // Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane i receives lane
 // i-1 (resp. i+1); the lane whose source is outside the wave keeps 'edge'.
@@ -55,20 +46,45 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
     const int x = x0 + lane;
     const int hxl = lane == 0 ? x - 1 : (lane == 63 ? x + 1 : x);  // x-halo column
 
+    // A lane's cells sit at the same (x, y) in every plane: their in-plane
+    // byte offsets (clamped into the plane, so loads are never predicated)
+    // and in-plane validity are computed once; per plane only the
+    // wave-uniform plane base and z validity change, so a load is one
+    // instruction with a scalar base and the lane's 32-bit offset.
+    const int xc = min(max(x, 0), nx - 1), hxc = min(max(hxl, 0), nx - 1);
+    const bool xok = (unsigned)x < (unsigned)nx, hxok = (unsigned)hxl < (unsigned)nx;
+    unsigned coff[kTY + 2], hoff[kTY];
+    unsigned cok = 0, hok = 0;  // bit j: row j of col / hal lies in the plane
+#pragma unroll
+    for (int j = 0; j < kTY + 2; j++) {
+        const int y = y0 - 1 + j;
+        coff[j] = (unsigned)(min(max(y, 0), ny - 1) * nx + xc) * (unsigned)sizeof(float);
+        cok |= (xok && (unsigned)y < (unsigned)ny) ? 1u << j : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kTY; j++) {
+        const int y = y0 + j;
+        hoff[j] = (unsigned)(min(max(y, 0), ny - 1) * nx + hxc) * (unsigned)sizeof(float);
+        hok |= (hxok && (unsigned)y < (unsigned)ny) ? 1u << j : 0u;
+    }
+    const size_t plane = (size_t)nx * ny;
+
     // slot(zz) = (zz - z0 + 1) & (kP-1); col[j] is row y0-1+j
     float col[kTY + 2][kP];
     float hal[kTY][kP];
     auto load_plane = [&](int zz, int slot) {
+        const char *p = reinterpret_cast<const char *>(in + (size_t)min(max(zz, 0), nz - 1) * plane);
 #pragma unroll
-        for (int j = 0; j < kTY + 2; j++) col[j][slot] = ld_cell(in, x, y0 - 1 + j, zz, nx, ny, nz);
+        for (int j = 0; j < kTY + 2; j++) col[j][slot] = *reinterpret_cast<const float *>(p + coff[j]);
 #pragma unroll
-        for (int j = 0; j < kTY; j++) hal[j][slot] = ld_cell(in, hxl, y0 + j, zz, nx, ny, nz);
+        for (int j = 0; j < kTY; j++) hal[j][slot] = *reinterpret_cast<const float *>(p + hoff[j]);
     };
     auto fix_plane = [&](int zz, int slot) {
+        const bool zok = (unsigned)zz < (unsigned)nz;  // wave-uniform
 #pragma unroll
-        for (int j = 0; j < kTY + 2; j++) col[j][slot] = cell_ok(x, y0 - 1 + j, zz, nx, ny, nz) ? col[j][slot] : nv;
+        for (int j = 0; j < kTY + 2; j++) col[j][slot] = (zok && ((cok >> j) & 1u)) ? col[j][slot] : nv;
 #pragma unroll
-        for (int j = 0; j < kTY; j++) hal[j][slot] = cell_ok(hxl, y0 + j, zz, nx, ny, nz) ? hal[j][slot] : nv;
+        for (int j = 0; j < kTY; j++) hal[j][slot] = (zok && ((hok >> j) & 1u)) ? hal[j][slot] : nv;
     };
 #pragma unroll
     for (int k = 0; k < kP - 1; k++) load_plane(z0 - 1 + k, k);
@@ -87,6 +103,7 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
             // plane z+1 is used for the first time now
             load_plane(z + 2, (u + 3) & (kP - 1));
             fix_plane(z + 1, sa);
+            char *op = reinterpret_cast<char *>(out + (size_t)z * plane);
 #pragma unroll
             for (int j = 0; j < kTY; j++) {
                 const float c = col[j + 1][sc];
@@ -98,9 +115,10 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
                 v = __fadd_rn(v, col[j + 1][sb]);             // z-1
                 v = __fadd_rn(v, col[j + 1][sa]);             // z+1
                 v = __fdiv_rn(v, 6.0f);
-                const int y = y0 + j;
-                if (z < z1 && x < nx && y < ny) {
-                    float *o = out + ((size_t)z * ny + y) * nx + x;
+                // row j+1 of col is output row y0+j: in the plane exactly
+                // when its cell is, and then coff is its unclamped offset
+                if (z < z1 && ((cok >> (j + 1)) & 1u)) {
+                    float *o = reinterpret_cast<float *>(op + coff[j + 1]);
                     if constexpr (NT)
                         __builtin_nontemporal_store(v, o);
                     else

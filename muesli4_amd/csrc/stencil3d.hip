@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, con
                                                     int ny, int nz, float nv, int zseg, int xtiles, int ytiles,
                                                     int nitems, float *__restrict__ maxdiff) {
     const int lane = threadIdx.x & 63;
-    const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const int item = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     float dmax = 0.0f;
     if (item < nitems) {
         const int xt = item % xtiles;
@@ -80,6 +80,7 @@ int heat_step(float *out, const float *in, int nx, int ny, int nz, float nv, flo
     if (rc) return rc;
     MSL_REQUIRE(nx >= 0 && ny >= 0 && nz >= 0, "msl_heat7_f32: negative shape");
     MSL_REQUIRE((int64_t)nx * ny * nz <= ((int64_t)1 << 40), "msl_heat7_f32: cube too large");
+    MSL_REQUIRE((int64_t)nx * ny < ((int64_t)1 << 30), "msl_heat7_f32: plane of 2^30 cells or more");
     if (nx == 0 || ny == 0 || nz == 0) return MSL_OK;
     MSL_REQUIRE(out && in && out != in, "msl_heat7_f32: null or aliased buffers");
     HeatShape s = heat_shape(nx, ny, nz);
@@ -138,6 +139,8 @@ int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float 
     if (rc) return rc;
     MSL_REQUIRE(iters >= 0 && residual_every >= 0, "msl_heat7_iterate_f32: negative count");
     MSL_REQUIRE(nx >= 0 && ny >= 0 && nz >= 0, "msl_heat7_iterate_f32: negative shape");
+    MSL_REQUIRE((int64_t)nx * ny < ((int64_t)1 << 30) && (int64_t)nx * ny * nz <= ((int64_t)1 << 40),
+                "msl_heat7_iterate_f32: cube too large");
     const int nres = residual_every > 0 ? iters / residual_every : 0;
     MSL_REQUIRE(nres == 0 || h_residuals != nullptr, "msl_heat7_iterate_f32: residual output required");
     if (h_result_in_b) *h_result_in_b = iters & 1;
