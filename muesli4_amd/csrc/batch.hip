@@ -515,7 +515,7 @@ int msl_batch_create(const msl_batch_grid *grids, int ngrids, void **h_batch) {
     for (int g = 0; g < ngrids; g++) {
         const msl_batch_grid &G = grids[g];
         MSL_REQUIRE(G.rows >= 0 && G.cols >= 0, "msl_batch_create: negative shape");
-        MSL_REQUIRE((int64_t)G.rows * G.cols <= INT32_MAX, "msl_batch_create: grid too large");
+        MSL_REQUIRE((int64_t)G.rows * G.cols <= INT32_MAX && G.cols < (1 << 30), "msl_batch_create: grid too large");
         MSL_REQUIRE(G.r >= 1 && G.r <= kBatchMaxR, "msl_batch_create: radius must be in [1, 10]");
         if (G.rows > 0 && G.cols > 0) {
             MSL_REQUIRE(G.a && G.b && G.a != G.b, "msl_batch_create: null or aliased grid buffers");

@@ -344,12 +344,14 @@ static size_t job_bytes(const msl_job &j) {
 static int validate_job(const msl_job &j, size_t budget) {
     switch (j.kind) {
         case MSL_JOB_BLUR2D:
-            MSL_REQUIRE(j.d0 >= 1 && j.d1 >= 1 && (int64_t)j.d0 * j.d1 <= INT32_MAX, "msl_job: bad 2-D shape");
+            MSL_REQUIRE(j.d0 >= 1 && j.d1 >= 1 && (int64_t)j.d0 * j.d1 <= INT32_MAX && j.d1 < (1 << 30),
+                        "msl_job: bad 2-D shape");
             MSL_REQUIRE(j.r >= 1 && j.r <= kBatchMaxR, "msl_job: radius must be in [1, 10]");
             MSL_REQUIRE(j.steps >= 1, "msl_job: steps must be >= 1");
             break;
         case MSL_JOB_HEAT3D:
-            MSL_REQUIRE(j.d0 >= 1 && j.d1 >= 1 && j.d2 >= 1 && (int64_t)j.d0 * j.d1 * j.d2 <= INT32_MAX,
+            MSL_REQUIRE(j.d0 >= 1 && j.d1 >= 1 && j.d2 >= 1 && (int64_t)j.d0 * j.d1 * j.d2 <= INT32_MAX &&
+                            (int64_t)j.d0 * j.d1 < (1 << 30),
                         "msl_job: bad 3-D shape");
             MSL_REQUIRE(j.steps >= 1, "msl_job: steps must be >= 1");
             break;

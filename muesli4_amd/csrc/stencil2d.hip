@@ -36,6 +36,9 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     __shared__ __attribute__((aligned(16))) T lds[4][2][StripConsts<R, C>::LDSW];
 
     const int wave = threadIdx.x >> 6;
+    // (the item is wave-uniform, but telling the compiler so with
+    // readfirstlane costs r = 10 5 %: the scalar path then lengthens the
+    // dependent address chain of each row)
     const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
     if (item >= nitems) return;
     const int sx = item % strips_x;
@@ -139,7 +142,7 @@ int mean_step(float *out, const float *in, int rows, int cols, int r, float nv, 
     if (rc) return rc;
     MSL_REQUIRE(rows >= 0 && cols >= 0, "msl_stencil2d_mean_f32: negative shape");
     MSL_REQUIRE(r >= 1 && r <= kMaxMeanR, "msl_stencil2d_mean_f32: radius must be in [1, 16]");
-    MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX, "msl_stencil2d_mean_f32: grid too large");
+    MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX && cols < (1 << 30), "msl_stencil2d_mean_f32: grid too large");
     if (rows == 0 || cols == 0) return MSL_OK;
     MSL_REQUIRE(out && in && out != in, "msl_stencil2d_mean_f32: null or aliased buffers");
     return kMeanLaunchers[r](out, in, rows, cols, nv, seg);
@@ -149,7 +152,7 @@ int gol_step(int32_t *out, const int32_t *in, int rows, int cols) {
     int rc = ensure_ready();
     if (rc) return rc;
     MSL_REQUIRE(rows >= 0 && cols >= 0, "msl_gol_i32: negative shape");
-    MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX, "msl_gol_i32: grid too large");
+    MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX && cols < (1 << 30), "msl_gol_i32: grid too large");
     if (rows == 0 || cols == 0) return MSL_OK;
     MSL_REQUIRE(out && in && out != in, "msl_gol_i32: null or aliased buffers");
     LifePolicy pol;

@@ -93,14 +93,20 @@ __device__ __forceinline__ VecN<T, C> load_dword_aligned(const T *p) {
 // kUnaligned mode each column picks its element of the clamped vector).
 // The march issues the load D rows ahead and applies the fix-up only when
 // the row is used, so the prefetched loads are never waited on early.
+// The row base is wave-uniform and the in-row offset a 32-bit byte offset
+// (rows of fewer than 2^30 cells), so a load takes a scalar base and one
+// per-lane offset.
 template <typename T, int C, int MODE>
 __device__ __forceinline__ VecN<T, C> load_vec(const T *__restrict__ in, int yy, int c, int rows, int cols) {
     const int yc = min(max(yy, 0), rows - 1);
     const T *rowp = in + (size_t)yc * cols;
-    if constexpr (MODE == kAligned) {
-        return load_aligned<T, C>(rowp + min(max(c, 0), cols - C));
-    } else if constexpr (MODE == kUnaligned) {
-        return load_dword_aligned<T, C>(rowp + min(max(c, 0), cols - C));
+    if constexpr (MODE == kAligned || MODE == kUnaligned) {
+        const unsigned off = (unsigned)min(max(c, 0), cols - C) * (unsigned)sizeof(T);
+        const T *p = reinterpret_cast<const T *>(reinterpret_cast<const char *>(rowp) + off);
+        if constexpr (MODE == kAligned)
+            return load_aligned<T, C>(p);
+        else
+            return load_dword_aligned<T, C>(p);
     } else {
         VecN<T, C> q;
 #pragma unroll
@@ -148,10 +154,12 @@ __device__ __forceinline__ void write_vec(T *__restrict__ orow, int x0, int cols
     typedef typename NativeVecN<T, C>::type V;
     if constexpr (MODE == kAligned) {
         if (x0 < cols) {
+            // scalar row base plus a 32-bit per-lane byte offset
+            V *p = reinterpret_cast<V *>(reinterpret_cast<char *>(orow) + (unsigned)x0 * (unsigned)sizeof(T));
             if constexpr (NT)
-                __builtin_nontemporal_store(to_native<T, C>(o), reinterpret_cast<V *>(orow + x0));
+                __builtin_nontemporal_store(to_native<T, C>(o), p);
             else
-                *reinterpret_cast<V *>(orow + x0) = to_native<T, C>(o);
+                *p = to_native<T, C>(o);
         }
     } else if constexpr (MODE == kUnaligned) {
         if (x0 + C <= cols) {
