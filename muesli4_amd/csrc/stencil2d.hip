@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "strip.hpp"
 
@@ -160,6 +161,83 @@ int gol_step(int32_t *out, const int32_t *in, int rows, int cols) {
     return launch_strip(out, in, rows, cols, (int32_t)0, pol, 0, "msl_gol_i32");
 }
 
+// ---- Game of Life, several generations per launch ---------------------------
+// A block owns a 64-column x (4 * RPW)-row window of the grid (cells outside
+// the grid read 0): lane l holds column l, wave w rows [w*RPW, (w+1)*RPW)
+// in registers.  Each generation exchanges the waves' first and last rows
+// through LDS, forms vertical 3-sums in registers and the horizontal ones by
+// DPP lane shifts, and applies B3/S23; cells outside the grid stay 0, as in
+// the one-step kernel.  The window edges lack neighbours, so after s
+// generations only cells at least s from every window edge are exact; the
+// block stores the central (64 - 2K) x (4*RPW - 2K) tile, exact after up to
+// K generations.  Integer rule: bit-identical to s single steps.
+__device__ __forceinline__ int32_t lane_from_left(int32_t v) {  // lane l gets lane l-1 (lane 0: 0)
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int32_t lane_from_right(int32_t v) {  // lane l gets lane l+1 (lane 63: 0)
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+}
+
+template <int K, int RPW>
+__global__ __launch_bounds__(256) void gol_multi_kernel(int32_t *__restrict__ out, const int32_t *__restrict__ in,
+                                                        int rows, int cols, int steps, int tiles_x) {
+    constexpr int H = 4 * RPW, TW = 64 - 2 * K, TH = H - 2 * K;
+    __shared__ int32_t edge[4][2][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int gx = tx * TW - K + lane;
+    const int gy0 = ty * TH - K + wave * RPW;
+    const bool colok = (unsigned)gx < (unsigned)cols;
+    int32_t c[RPW + 2];  // rows gy0-1 .. gy0+RPW
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int gy = gy0 + i;
+        c[i + 1] = (colok && (unsigned)gy < (unsigned)rows) ? in[(size_t)gy * cols + gx] : 0;
+    }
+    for (int s = 0; s < steps; s++) {
+        edge[wave][0][lane] = c[1];
+        edge[wave][1][lane] = c[RPW];
+        __syncthreads();
+        c[0] = wave > 0 ? edge[wave - 1][1][lane] : 0;
+        c[RPW + 1] = wave < 3 ? edge[wave + 1][0][lane] : 0;
+        __syncthreads();
+        int32_t nw[RPW];
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const int32_t vs = c[i] + c[i + 1] + c[i + 2];
+            const int32_t live = c[i + 1];
+            const int32_t sum = lane_from_left(vs) + vs + lane_from_right(vs) - live;
+            // B3/S23 as in examples/gameoflife.cpp:18-43
+            int32_t next = 0;
+            if (live == 1 && (sum == 2 || sum == 3)) next = 1;
+            if (live == 0 && sum == 3) next = 1;
+            const int gy = gy0 + i;
+            nw[i] = (colok && (unsigned)gy < (unsigned)rows) ? next : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < RPW; i++) c[i + 1] = nw[i];
+    }
+    if (lane >= K && lane < 64 - K && colok) {
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const int wy = wave * RPW + i, gy = gy0 + i;
+            if (wy >= K && wy < H - K && gy < rows) out[(size_t)gy * cols + gx] = c[i + 1];
+        }
+    }
+}
+
+constexpr int kGolSteps = 4;  // generations per launch
+constexpr int kGolRows = 8;   // rows per wave: 64 x 32 windows, 56 x 24 tiles
+
+static int gol_multi_step(int32_t *out, const int32_t *in, int rows, int cols, int steps) {
+    constexpr int TW = 64 - 2 * kGolSteps, TH = 4 * kGolRows - 2 * kGolSteps;
+    const int tiles_x = (cols + TW - 1) / TW;
+    const long blocks = (long)tiles_x * ((rows + TH - 1) / TH);
+    MSL_REQUIRE(blocks <= INT32_MAX, "msl_gol_iterate_i32: grid too large");
+    gol_multi_kernel<kGolSteps, kGolRows><<<(int)blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, steps, tiles_x);
+    return check_launch("msl_gol_iterate_i32");
+}
+
 // ---- exhaustive check of div_odd_square against IEEE division -------------
 __global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
@@ -295,12 +373,29 @@ int msl_gol_iterate_i32(int32_t *d_a, int32_t *d_b, int rows, int cols, int iter
     MSL_REQUIRE(rows >= 0 && cols >= 0, "msl_gol_iterate_i32: negative shape");
     if (h_result_in_b) *h_result_in_b = iters & 1;
     if (iters == 0 || rows == 0 || cols == 0) return MSL_OK;
+    MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX, "msl_gol_iterate_i32: grid too large");
+    MSL_REQUIRE(d_a && d_b && d_a != d_b, "msl_gol_iterate_i32: null or aliased buffers");
+    // Launches of up to kGolSteps generations each, alternating buffers.  The
+    // result must end in b exactly when iters is odd (as with one launch per
+    // generation), so the number of launches must have the parity of iters:
+    // when it does not, one launch of several generations is split in two
+    // (there is one, since launches of one generation each number iters).
+    std::vector<int> plan;
+    for (int done = 0; done < iters; done += plan.back()) plan.push_back(std::min(kGolSteps, iters - done));
+    if ((int)(plan.size() & 1) != (iters & 1)) {
+        for (size_t k = plan.size(); k-- > 0;)
+            if (plan[k] >= 2) {
+                plan[k] -= 1;
+                plan.insert(plan.begin() + k + 1, 1);
+                break;
+            }
+    }
     std::string key = graph_key("gol", d_a, d_b, rows, cols, iters, 0, 0);
     return run_cached_graph(key, [&]() {
-        for (int it = 0; it < iters; it++) {
-            int32_t *src = (it & 1) ? d_b : d_a;
-            int32_t *dst = (it & 1) ? d_a : d_b;
-            int brc = gol_step(dst, src, rows, cols);
+        for (size_t k = 0; k < plan.size(); k++) {
+            int32_t *src = (k & 1) ? d_b : d_a;
+            int32_t *dst = (k & 1) ? d_a : d_b;
+            int brc = gol_multi_step(dst, src, rows, cols, plan[k]);
             if (brc) return brc;
         }
         return MSL_OK;

@@ -268,6 +268,29 @@ def test_gol_1024_100_iterations_and_fold():
     assert s.value == int(ref.sum())
 
 
+# The iterate entry applies up to four generations per launch in LDS
+# (partial tiles at the edges, launch plans split to keep the result-buffer
+# parity); every count and ragged shape is bit-exact against the oracle.
+@pytest.mark.parametrize("shape", [(3, 5), (17, 33), (100, 260), (33, 64), (1, 40), (40, 1), (70, 70)])
+@pytest.mark.parametrize("iters", [1, 2, 3, 4, 5, 6, 9, 13])
+def test_gol_iterate_multi_generation(shape, iters):
+    rows, cols = shape
+    note(f"gol iterate {shape} x{iters}")
+    n = rows * cols
+    a = DeviceArray((rows, cols), np.int32)
+    b = DeviceArray((rows, cols), np.int32)
+    ref = R.fill_bernoulli_i32(n, 77).reshape(rows, cols)
+    for _ in range(iters):
+        ref = R.gol_i32(ref)
+    inb = C.c_int()
+    for rep in range(2):  # the second call replays the cached graph
+        call("msl_fill_bernoulli_i32", a.ptr, n, 77, R.BERNOULLI_03)
+        call("msl_gol_iterate_i32", a.ptr, b.ptr, rows, cols, iters, C.byref(inb))
+        assert inb.value == (iters & 1)
+        np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(rows, cols), ref,
+                                      err_msg=f"replay {rep}")
+
+
 # Small grids march 64-column strips; grids with enough 256-column strip
 # segments (on a 256-CU device) march 256-column strips in the aligned,
 # unaligned-row (cols % 4 != 0) and scalar (cols < 4) row modes.
