@@ -229,6 +229,93 @@ __global__ __launch_bounds__(256) void gol_multi_kernel(int32_t *__restrict__ ou
 constexpr int kGolSteps = 4;  // generations per launch
 constexpr int kGolRows = 8;   // rows per wave: 64 x 32 windows, 56 x 24 tiles
 
+// Mean blur of radius 1, several generations per launch, laid out like the
+// Game of Life kernel above: lane = column, wave = RPW rows, row sums
+// ((0 + left) + centre) + right with the side cells by DPP, column sums
+// ((0 + above) + row) + below with the wave edge rows through LDS, then the
+// exact division by 9 -- the operation order of the one-step kernel, so
+// every generation is bit-identical to it.  Cells outside the grid hold nv
+// at every generation.
+template <int K, int RPW>
+__global__ __launch_bounds__(256) void mean3_multi_kernel(float *__restrict__ out, const float *__restrict__ in,
+                                                          int rows, int cols, float nv, int steps, int tiles_x,
+                                                          float denom, float rcp) {
+    constexpr int H = 4 * RPW, TW = 64 - 2 * K, TH = H - 2 * K;
+    __shared__ float edge[4][2][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int gx = tx * TW - K + lane;
+    const int gy0 = ty * TH - K + wave * RPW;
+    const bool colok = (unsigned)gx < (unsigned)cols;
+    float c[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int gy = gy0 + i;
+        c[i] = (colok && (unsigned)gy < (unsigned)rows) ? in[(size_t)gy * cols + gx] : nv;
+    }
+    for (int s = 0; s < steps; s++) {
+        float rs[RPW + 2];  // row sums of rows gy0-1 .. gy0+RPW
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const float l = __int_as_float(lane_from_left(__float_as_int(c[i])));
+            const float r = __int_as_float(lane_from_right(__float_as_int(c[i])));
+            rs[i + 1] = __fadd_rn(__fadd_rn(__fadd_rn(0.0f, l), c[i]), r);
+        }
+        edge[wave][0][lane] = rs[1];
+        edge[wave][1][lane] = rs[RPW];
+        __syncthreads();
+        rs[0] = wave > 0 ? edge[wave - 1][1][lane] : 0.0f;
+        rs[RPW + 1] = wave < 3 ? edge[wave + 1][0][lane] : 0.0f;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const float t = __fadd_rn(__fadd_rn(__fadd_rn(0.0f, rs[i]), rs[i + 1]), rs[i + 2]);
+            const int gy = gy0 + i;
+            c[i] = (colok && (unsigned)gy < (unsigned)rows) ? div_odd_square(t, denom, rcp) : nv;
+        }
+    }
+    if (lane >= K && lane < 64 - K && colok) {
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const int wy = wave * RPW + i, gy = gy0 + i;
+            if (wy >= K && wy < H - K && gy < rows) out[(size_t)gy * cols + gx] = c[i];
+        }
+    }
+}
+
+constexpr int kMean3Steps = 6;  // generations per launch of the radius-1 mean blur
+constexpr int kMean3Rows = 16;  // rows per wave: 64 x 64 windows, 52 x 52 tiles
+
+// Launch plan for `iters` generations of at most k per launch, alternating
+// buffers: the result must end in b exactly when iters is odd (as with one
+// launch per generation), so the number of launches must have the parity of
+// iters; when it does not, one launch of several generations is split in
+// two (there is one, since launches of one generation each number iters).
+static std::vector<int> generation_plan(int iters, int k) {
+    std::vector<int> plan;
+    for (int done = 0; done < iters; done += plan.back()) plan.push_back(std::min(k, iters - done));
+    if ((int)(plan.size() & 1) != (iters & 1)) {
+        for (size_t j = plan.size(); j-- > 0;)
+            if (plan[j] >= 2) {
+                plan[j] -= 1;
+                plan.insert(plan.begin() + j + 1, 1);
+                break;
+            }
+    }
+    return plan;
+}
+
+static int mean3_multi_step(float *out, const float *in, int rows, int cols, float nv, int steps) {
+    constexpr int TW = 64 - 2 * kMean3Steps, TH = 4 * kMean3Rows - 2 * kMean3Steps;
+    const int tiles_x = (cols + TW - 1) / TW;
+    const long blocks = (long)tiles_x * ((rows + TH - 1) / TH);
+    MSL_REQUIRE(blocks <= INT32_MAX, "msl_stencil2d_mean_iterate_f32: grid too large");
+    const MeanPolicy<1> pol = MeanPolicy<1>::make();
+    mean3_multi_kernel<kMean3Steps, kMean3Rows><<<(int)blocks, 256, 0, ctx().stream>>>(
+        out, in, rows, cols, nv, steps, tiles_x, pol.denom, pol.rcp);
+    return check_launch("msl_stencil2d_mean_iterate_f32");
+}
+
 static int gol_multi_step(int32_t *out, const int32_t *in, int rows, int cols, int steps) {
     constexpr int TW = 64 - 2 * kGolSteps, TH = 4 * kGolRows - 2 * kGolSteps;
     const int tiles_x = (cols + TW - 1) / TW;
@@ -331,7 +418,22 @@ int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, i
         int32_t i;
     } nvb;
     nvb.f = nv;
+    MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX && cols < (1 << 30), "msl_stencil2d_mean_iterate_f32: grid too large");
+    MSL_REQUIRE(d_a && d_b && d_a != d_b, "msl_stencil2d_mean_iterate_f32: null or aliased buffers");
     std::string key = graph_key("mean2d", d_a, d_b, rows, cols, r, nvb.i, iters);
+    if (r == 1) {  // several generations per launch
+        const std::vector<int> plan = generation_plan(iters, kMean3Steps);
+        return run_cached_graph(key, [&]() {
+            for (size_t k = 0; k < plan.size(); k++) {
+                float *src = (k & 1) ? d_b : d_a;
+                float *dst = (k & 1) ? d_a : d_b;
+                int brc = plan[k] == 1 ? mean_step(dst, src, rows, cols, r, nv, 0)
+                                       : mean3_multi_step(dst, src, rows, cols, nv, plan[k]);
+                if (brc) return brc;
+            }
+            return MSL_OK;
+        });
+    }
     return run_cached_graph(key, [&]() {
         for (int it = 0; it < iters; it++) {
             float *src = (it & 1) ? d_b : d_a;
@@ -375,21 +477,7 @@ int msl_gol_iterate_i32(int32_t *d_a, int32_t *d_b, int rows, int cols, int iter
     if (iters == 0 || rows == 0 || cols == 0) return MSL_OK;
     MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX, "msl_gol_iterate_i32: grid too large");
     MSL_REQUIRE(d_a && d_b && d_a != d_b, "msl_gol_iterate_i32: null or aliased buffers");
-    // Launches of up to kGolSteps generations each, alternating buffers.  The
-    // result must end in b exactly when iters is odd (as with one launch per
-    // generation), so the number of launches must have the parity of iters:
-    // when it does not, one launch of several generations is split in two
-    // (there is one, since launches of one generation each number iters).
-    std::vector<int> plan;
-    for (int done = 0; done < iters; done += plan.back()) plan.push_back(std::min(kGolSteps, iters - done));
-    if ((int)(plan.size() & 1) != (iters & 1)) {
-        for (size_t k = plan.size(); k-- > 0;)
-            if (plan[k] >= 2) {
-                plan[k] -= 1;
-                plan.insert(plan.begin() + k + 1, 1);
-                break;
-            }
-    }
+    const std::vector<int> plan = generation_plan(iters, kGolSteps);
     std::string key = graph_key("gol", d_a, d_b, rows, cols, iters, 0, 0);
     return run_cached_graph(key, [&]() {
         for (size_t k = 0; k < plan.size(); k++) {

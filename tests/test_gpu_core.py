@@ -248,6 +248,26 @@ def test_mean_blur_iterate_graph(r, iters):
         np.testing.assert_array_equal(got, ref, err_msg=f"replay {rep}")
 
 
+# Radius 1 iterates up to four generations per launch in registers; every
+# count, shape (partial tiles) and neutral value is bit-exact.
+@pytest.mark.parametrize("shape", [(3, 5), (17, 33), (100, 260), (33, 64), (1, 40), (40, 1), (70, 70)])
+@pytest.mark.parametrize("iters", [2, 3, 4, 5, 8, 9])
+def test_mean_blur_r1_iterate_multi_generation(shape, iters):
+    rows, cols = shape
+    n = rows * cols
+    for nv in (0.0, 3.5):
+        note(f"mean r=1 iterate {shape} x{iters} nv={nv}")
+        a = dev_uniform(n, 55)
+        b = DeviceArray((rows, cols), np.float32)
+        ref = R.fill_uniform_f32(n, 55).reshape(rows, cols)
+        for _ in range(iters):
+            ref = R.stencil2d_mean_f32(ref, 1, nv)
+        inb = C.c_int()
+        call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, 1, nv, iters, C.byref(inb))
+        assert inb.value == (iters & 1)
+        np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(rows, cols), ref)
+
+
 def test_gol_1024_100_iterations_and_fold():
     rows = cols = 1024
     iters = 100
