@@ -51,7 +51,8 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
     ap.add_argument("--kernel", default=None,
                     help="run one kernel only, --reps times, for profiler counter runs: "
-                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | heat7 | gol | batch")
+                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | blur_r1_multi | heat7 | gol | "
+                         "gol_multi | batch")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     if args.kernel:
@@ -182,8 +183,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    kname = ("mean3_multi_kernel<6, 16> (six generations per launch)" if dom_r == 1 else
+             f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true>")
     roofline = {"bound": "hbm",
-                "kernel": f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true> (8192x8192 ping-pong, 8 B/cell)",
+                "kernel": f"{kname} (8192x8192 ping-pong, 8 B/cell per iteration)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": 8 * n, "avg_launch_ms": round(launch_ms, 5)}
@@ -256,15 +259,19 @@ def run_single_kernel(name, reps):
         bufs = [a, b]
         for i in range(reps + 2):
             call("msl_heat7_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, nx, ny, nz, 0.0)
-    elif name == "gol":
+    elif name in ("gol", "gol_multi"):
         rows = cols = 1024
         n = rows * cols
         a = DeviceArray(n, np.int32)
         b = DeviceArray(n, np.int32)
         call("msl_fill_bernoulli_i32", a.ptr, n, SEED, int(0.3 * 2.0 ** 64))
         bufs = [a, b]
+        inb = C.c_int()
         for i in range(reps + 2):
-            call("msl_gol_i32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, rows, cols)
+            if name == "gol":
+                call("msl_gol_i32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, rows, cols)
+            else:  # one launch of four generations per call
+                call("msl_gol_iterate_i32", a.ptr, b.ptr, rows, cols, 4, C.byref(inb))
     else:
         n = ROWS * COLS
         a = DeviceArray(n, np.float32)
@@ -282,6 +289,9 @@ def run_single_kernel(name, reps):
                 call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n)
             elif name == "fold":
                 call("msl_fold_sum_f32", a.ptr, n, d_res.ptr)
+            elif name == "blur_r1_multi":  # one launch of six radius-1 generations per call
+                inb = C.c_int()
+                call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, 1, 0.0, 6, C.byref(inb))
             elif name.startswith("blur_r"):
                 r = int(name[len("blur_r"):])
                 call("msl_stencil2d_mean_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, ROWS, COLS, r, 0.0)
