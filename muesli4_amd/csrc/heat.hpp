@@ -134,4 +134,116 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
     }
 }
 
+// Two generations per pass.  The wave's 64 lanes cover columns x0-2 ..
+// x0+61 and it keeps input rows y0-2 .. y0+kTY+1; generation 1 is computed
+// on rows y0-1 .. y0+kTY (exact on lanes 1 .. 62), generation 2 from it on
+// rows y0 .. y0+kTY-1, exact on lanes 2 .. 61 (columns x0 .. x0+59), which
+// are stored: the two edge lanes on each side serve as x-halo, so no extra
+// halo loads are needed.  Along z generation 1 of plane z is formed first,
+// then generation 2 of plane z-1 from the ring of generation-1 planes.
+// Generation-1 cells outside the cube are nv, as the second step reads them.
+// Every cell is computed in the one-step order, so the result is
+// bit-identical to two single steps.  kResidual: max |generation 2 -
+// generation 1| into *dmax.
+constexpr int kT2X = kTX - 4;  // output columns of a two-generation wave tile
+
+template <bool kResidual, bool NT = true>
+__device__ __forceinline__ void heat7x2_march(float *__restrict__ out, const float *__restrict__ in, int nx, int ny,
+                                              int nz, float nv, int x0, int y0, int z0, int z1, float *dmax) {
+    constexpr int NI = kTY + 4;  // input rows y0-2 .. y0+kTY+1
+    constexpr int N1 = kTY + 2;  // generation-1 rows y0-1 .. y0+kTY
+    const int lane = threadIdx.x & 63;
+    const int x = x0 - 2 + lane;
+    const int xc = min(max(x, 0), nx - 1);
+    const bool xok = (unsigned)x < (unsigned)nx;
+    unsigned ioff[NI];
+    unsigned iok = 0;  // bit j: input row j lies in the plane
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        const int y = y0 - 2 + j;
+        ioff[j] = (unsigned)(min(max(y, 0), ny - 1) * nx + xc) * (unsigned)sizeof(float);
+        iok |= (xok && (unsigned)y < (unsigned)ny) ? 1u << j : 0u;
+    }
+    const size_t plane = (size_t)nx * ny;
+
+    // input slot(zz) = (zz - z0 + 2) & 3; generation-1 slot(zz) = (zz - z0 + 1) & 3
+    float inp[NI][kP], g1[N1][kP];
+    auto load_plane = [&](int zz, int slot) {
+        const char *p = reinterpret_cast<const char *>(in + (size_t)min(max(zz, 0), nz - 1) * plane);
+#pragma unroll
+        for (int j = 0; j < NI; j++) inp[j][slot] = *reinterpret_cast<const float *>(p + ioff[j]);
+    };
+    auto fix_plane = [&](int zz, int slot) {
+        const bool zok = (unsigned)zz < (unsigned)nz;
+#pragma unroll
+        for (int j = 0; j < NI; j++) inp[j][slot] = (zok && ((iok >> j) & 1u)) ? inp[j][slot] : nv;
+    };
+    // one step of the 7-point rule on a lane's column: rows r-1, r, r+1 of
+    // plane slots (below, centre, above), x neighbours from the adjacent lanes
+    auto cell = [&](float(&a)[kP], float(&m)[kP], float(&b)[kP], float(&zl)[kP], float(&zh)[kP], int sb, int sc,
+                    int sa) -> float {
+        (void)sb;
+        (void)sa;
+        const float c = m[sc];
+        float v = 0.0f;
+        v = __fadd_rn(v, from_left(c, c));   // x-1
+        v = __fadd_rn(v, from_right(c, c));  // x+1
+        v = __fadd_rn(v, a[sc]);             // y-1
+        v = __fadd_rn(v, b[sc]);             // y+1
+        v = __fadd_rn(v, zl[sb]);            // z-1
+        v = __fadd_rn(v, zh[sa]);            // z+1
+        return __fdiv_rn(v, 6.0f);
+    };
+    // generation 1 of plane zi from input slots (zi-1, zi, zi+1) into g1 slot d
+    auto gen1 = [&](int zi, int sb, int sc, int sa, int d) {
+        const bool zok = (unsigned)zi < (unsigned)nz;
+#pragma unroll
+        for (int j = 0; j < N1; j++) {
+            const float v = cell(inp[j], inp[j + 1], inp[j + 2], inp[j + 1], inp[j + 1], sb, sc, sa);
+            g1[j][d] = (zok && ((iok >> (j + 1)) & 1u)) ? v : nv;
+        }
+    };
+
+#pragma unroll
+    for (int j = 0; j < N1; j++) g1[j][3] = nv;  // read (unstored) by the first step
+    load_plane(z0 - 2, 0);
+    load_plane(z0 - 1, 1);
+    load_plane(z0, 2);
+    load_plane(z0 + 1, 3);
+    fix_plane(z0 - 2, 0);
+    fix_plane(z0 - 1, 1);
+    fix_plane(z0, 2);
+    gen1(z0 - 1, 0, 1, 2, 0);
+
+    // step zi: generation 1 of plane zi, then generation 2 of plane zi-1
+    for (int zb = z0; zb <= z1; zb += kP) {
+#pragma unroll
+        for (int u = 0; u < kP; u++) {
+            const int zi = zb + u;
+            load_plane(zi + 2, u & 3);
+            fix_plane(zi + 1, (u + 3) & 3);
+            gen1(zi, (u + 1) & 3, (u + 2) & 3, (u + 3) & 3, (u + 1) & 3);
+            const int zo = zi - 1;
+            const int tb = (u + 3) & 3, tc = u & 3, ta = (u + 1) & 3;  // g1 slots of zo-1, zo, zo+1
+            char *op = reinterpret_cast<char *>(out + (size_t)min(max(zo, 0), nz - 1) * plane);
+#pragma unroll
+            for (int j = 0; j < kTY; j++) {
+                const float v = cell(g1[j], g1[j + 1], g1[j + 2], g1[j + 1], g1[j + 1], tb, tc, ta);
+                // output row y0+j is input row j+2: stored when in the cube
+                if (zo >= z0 && zo < z1 && lane >= 2 && lane <= 61 && ((iok >> (j + 2)) & 1u)) {
+                    float *o = reinterpret_cast<float *>(op + ioff[j + 2]);
+                    if constexpr (NT)
+                        __builtin_nontemporal_store(v, o);
+                    else
+                        *o = v;
+                    if constexpr (kResidual) {
+                        const float dd = fabsf(__fsub_rn(v, g1[j + 1][tc]));
+                        *dmax = dd > *dmax ? dd : *dmax;
+                    }
+                }
+            }
+        }
+    }
+}
+
 }  // namespace mslhip

@@ -11,9 +11,11 @@
 // reference functor's operation order.  The optional fused residual folds
 // max |v - in| into one device scalar with an integer atomic max (the bit
 // patterns of non-negative floats order like unsigned integers).
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "heat.hpp"
 
@@ -53,6 +55,38 @@ __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, con
     }
 }
 
+// Two generations per launch (heat7x2_march); tiles of kT2X x kTY columns.
+template <bool kResidual, bool NT>
+__global__ __launch_bounds__(256) void heat7x2_kernel(float *__restrict__ out, const float *__restrict__ in, int nx,
+                                                      int ny, int nz, float nv, int zseg, int xtiles, int ytiles,
+                                                      int nitems, float *__restrict__ maxdiff) {
+    const int lane = threadIdx.x & 63;
+    const int item = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    float dmax = 0.0f;
+    if (item < nitems) {
+        const int xt = item % xtiles;
+        const int rest = item / xtiles;
+        const int yt = rest % ytiles;
+        const int zs = rest / ytiles;
+        const int z0 = zs * zseg;
+        heat7x2_march<kResidual, NT>(out, in, nx, ny, nz, nv, xt * kT2X, yt * kTY, z0, min(z0 + zseg, nz), &dmax);
+    }
+    if (kResidual) {
+        __shared__ float wmax[4];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const float o = __shfl_xor(dmax, m);
+            dmax = o > dmax ? o : dmax;
+        }
+        if (lane == 0) wmax[threadIdx.x >> 6] = dmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+            atomicMax(reinterpret_cast<unsigned int *>(maxdiff), __float_as_uint(m));
+        }
+    }
+}
+
 struct HeatShape {
     int zseg, xtiles, ytiles, zsegs, nitems;
 };
@@ -73,6 +107,36 @@ inline HeatShape heat_shape(int nx, int ny, int nz) {
     s.zsegs = (nz + zseg - 1) / zseg;
     s.nitems = s.xtiles * s.ytiles * s.zsegs;
     return s;
+}
+
+// Cubes above this size run their iterations two generations per launch:
+// the pass reads and writes half the bytes per generation but computes 1.3x
+// the cells (tile halos), which pays where HBM or the Infinity Cache bounds
+// the single step (measured: 272^3 4.4 -> 5.4 TB/s, 320^3 5.0 -> 5.6 TB/s,
+// 512^3 4.9 -> 6.5 TB/s) and not on smaller cubes (256^3 5.6 -> 4.8 TB/s).
+constexpr size_t kHeatPairMinBytes = (size_t)64 << 20;
+
+// Two generations in one launch; maxdiff (if any) gets the second one's
+// max |change|.
+int heat_step2(float *out, const float *in, int nx, int ny, int nz, float nv, float *maxdiff) {
+    HeatShape s = heat_shape(nx, ny, nz);
+    s.xtiles = (nx + kT2X - 1) / kT2X;
+    s.nitems = s.xtiles * s.ytiles * s.zsegs;
+    const bool nt = stream_stores((size_t)nx * ny * nz * sizeof(float));
+    const int blocks = (s.nitems + 3) / 4;
+    if (maxdiff && nt)
+        heat7x2_kernel<true, true><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles,
+                                                                     s.ytiles, s.nitems, maxdiff);
+    else if (maxdiff)
+        heat7x2_kernel<true, false><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles,
+                                                                      s.ytiles, s.nitems, maxdiff);
+    else if (nt)
+        heat7x2_kernel<false, true><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles,
+                                                                      s.ytiles, s.nitems, nullptr);
+    else
+        heat7x2_kernel<false, false><<<blocks, 256, 0, ctx().stream>>>(out, in, nx, ny, nz, nv, s.zseg, s.xtiles,
+                                                                       s.ytiles, s.nitems, nullptr);
+    return check_launch("msl_heat7_iterate_f32");
 }
 
 int heat_step(float *out, const float *in, int nx, int ny, int nz, float nv, float *maxdiff) {
@@ -145,9 +209,15 @@ int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float 
     MSL_REQUIRE(nres == 0 || h_residuals != nullptr, "msl_heat7_iterate_f32: residual output required");
     if (h_result_in_b) *h_result_in_b = iters & 1;
     if (iters == 0 || nx == 0 || ny == 0 || nz == 0) return MSL_OK;
+    // generations per launch: MSL_HEAT_GENS = 1 or 2 forces (read on every
+    // call, so a process can compare both); otherwise pairs above
+    // kHeatPairMinBytes
+    const char *ge = getenv("MSL_HEAT_GENS");
+    const int gens_env = ge ? atoi(ge) : 0;
+    const bool pairs = gens_env == 2 || (gens_env != 1 && (size_t)nx * ny * nz * sizeof(float) > kHeatPairMinBytes);
     char keybuf[256];
-    snprintf(keybuf, sizeof keybuf, "heat7|%p|%p|%d|%d|%d|%08x|%d|%d", (void *)d_a, (void *)d_b, nx, ny, nz,
-             __builtin_bit_cast(unsigned, nv), iters, residual_every);
+    snprintf(keybuf, sizeof keybuf, "heat7|%p|%p|%d|%d|%d|%08x|%d|%d|%d", (void *)d_a, (void *)d_b, nx, ny, nz,
+             __builtin_bit_cast(unsigned, nv), iters, residual_every, pairs ? 2 : 1);
     std::string key(keybuf);
     HeatGraph hg;
     {
@@ -168,12 +238,30 @@ int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float 
             zero_f32_kernel<<<(nres + 255) / 256, 256, 0, ctx().stream>>>(d_res, nres);
             brc = check_launch("zero residuals");
         }
-        int k = 0;
-        for (int it = 0; it < iters && brc == MSL_OK; it++) {
-            float *src = (it & 1) ? d_b : d_a;
-            float *dst = (it & 1) ? d_a : d_b;
-            bool res = residual_every > 0 && ((it + 1) % residual_every == 0);
-            brc = heat_step(dst, src, nx, ny, nz, nv, res ? d_res + (k++) : nullptr);
+        // Launches of two generations where possible: a residual generation
+        // must be the last of its launch, and the number of launches must
+        // have the parity of iters (the result ends in b exactly when iters is
+        // odd, as with one launch per generation).
+        auto is_res = [&](int it) { return residual_every > 0 && ((it + 1) % residual_every == 0); };
+        std::vector<int> plan;
+        for (int it = 0; it < iters; it += plan.back())
+            plan.push_back((pairs && it + 1 < iters && !is_res(it)) ? 2 : 1);
+        if ((int)(plan.size() & 1) != (iters & 1)) {
+            for (size_t j = plan.size(); j-- > 0;)
+                if (plan[j] == 2) {
+                    plan[j] = 1;
+                    plan.insert(plan.begin() + j + 1, 1);
+                    break;
+                }
+        }
+        int k = 0, it = 0;
+        for (size_t j = 0; j < plan.size() && brc == MSL_OK; j++) {
+            float *src = (j & 1) ? d_b : d_a;
+            float *dst = (j & 1) ? d_a : d_b;
+            it += plan[j];
+            const bool res = is_res(it - 1);
+            brc = plan[j] == 2 ? heat_step2(dst, src, nx, ny, nz, nv, res ? d_res + (k++) : nullptr)
+                               : heat_step(dst, src, nx, ny, nz, nv, res ? d_res + (k++) : nullptr);
         }
         void *g = nullptr;
         int erc = msl_graph_end(&g);

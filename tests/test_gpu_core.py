@@ -391,6 +391,66 @@ def test_heat7_residual_above_store_threshold():
     assert float(md.to_host()[0]) == m
 
 
+# The iterate entry runs two generations per launch where it can (a residual
+# generation ends its launch); ragged cubes, counts and residual cadences are
+# bit-exact against single oracle steps, residuals included.
+@pytest.fixture
+def heat_pairs(monkeypatch):
+    """Force two generations per launch (the library's default above 64 MiB)."""
+    monkeypatch.setenv("MSL_HEAT_GENS", "2")
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 7, 9), (17, 9, 65), (9, 33, 130), (3, 70, 63), (20, 20, 124)])
+@pytest.mark.parametrize("iters,every", [(1, 0), (2, 0), (3, 0), (5, 0), (2, 1), (4, 2), (6, 3), (7, 2), (9, 4)])
+def test_heat7_iterate_two_generation_launches(shape, iters, every, heat_pairs):
+    nz, ny, nx = shape
+    note(f"heat7 iterate pairs {shape} x{iters} every {every}")
+    n = nx * ny * nz
+    for nv in (0.0, 0.25):
+        a = dev_uniform(n, 44, R.SCALE_UNIT)
+        b = DeviceArray(shape, np.float32)
+        nres = iters // every if every else 0
+        res = (C.c_float * max(nres, 1))()
+        inb = C.c_int()
+        ref = R.fill_uniform_f32(n, 44, R.SCALE_UNIT).reshape(shape)
+        ref_res = []
+        for it in range(iters):
+            if every and (it + 1) % every == 0:
+                ref, m = R.heat7_residual_f32(ref, nv)
+                ref_res.append(m)
+            else:
+                ref = R.heat7_f32(ref, nv)
+        call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, nv, iters, every, res if nres else None,
+             C.byref(inb))
+        assert inb.value == (iters & 1)
+        np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(shape), ref, err_msg=f"nv={nv}")
+        assert list(res)[:nres] == ref_res
+
+
+def test_heat7_iterate_large_cube_default_pairs():
+    """A 77 MiB cube takes the paired-generation launches by default."""
+    shape = (272, 272, 272)
+    nz, ny, nx = shape
+    iters, every = 5, 2
+    note(f"heat7 iterate default {shape} x{iters}")
+    n = nx * ny * nz
+    a = dev_uniform(n, 46, R.SCALE_UNIT)
+    b = DeviceArray(shape, np.float32)
+    res = (C.c_float * (iters // every))()
+    inb = C.c_int()
+    ref = R.fill_uniform_f32(n, 46, R.SCALE_UNIT).reshape(shape)
+    ref_res = []
+    for it in range(iters):
+        if (it + 1) % every == 0:
+            ref, m = R.heat7_residual_f32(ref, 0.0)
+            ref_res.append(m)
+        else:
+            ref = R.heat7_f32(ref, 0.0)
+    call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+    np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(shape), ref)
+    assert list(res) == ref_res
+
+
 def test_heat7_iterate_with_residuals():
     shape = (40, 48, 72)
     nz, ny, nx = shape
