@@ -51,7 +51,7 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
     ap.add_argument("--kernel", default=None,
                     help="run one kernel only, --reps times, for profiler counter runs: "
-                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | blur_r1_multi | heat7 | gol | "
+                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | blur_r1_multi | heat7 | heat7_pairs | gol | "
                          "gol_multi | batch")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
@@ -250,15 +250,19 @@ def run_single_kernel(name, reps):
         hip.synchronize()
         log(f"[bench] ran batch x{max(1, reps // BATCH_ITERS)} runs of {BATCH_ITERS} steps")
         return
-    if name == "heat7":
+    if name in ("heat7", "heat7_pairs"):
         nx = ny = nz = 512
         n = nx * ny * nz
         a = DeviceArray(n, np.float32)
         b = DeviceArray(n, np.float32)
         call("msl_fill_uniform_f32", a.ptr, n, SEED, float(np.float32(1.0 / 16777216.0)))
         bufs = [a, b]
+        inb = C.c_int()
         for i in range(reps + 2):
-            call("msl_heat7_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, nx, ny, nz, 0.0)
+            if name == "heat7":
+                call("msl_heat7_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, nx, ny, nz, 0.0)
+            else:  # two launches of two generations each per call (four keeps the buffer parity)
+                call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, 4, 0, None, C.byref(inb))
     elif name in ("gol", "gol_multi"):
         rows = cols = 1024
         n = rows * cols
