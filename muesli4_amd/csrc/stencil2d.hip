@@ -53,6 +53,28 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
                                              min(y0 + seg, rows), pol, lds[wave], nullptr, sy & 1);
 }
 
+// Two generations of the radius-R mean blur per wave item (strip2_march_dir):
+// output strips of 256 - 2RP columns, segments marched in alternating
+// directions as in strip_stencil_kernel.  Aligned rows only.
+template <int R, bool NT>
+__global__ __launch_bounds__(256) void strip2_mean_kernel(float *__restrict__ out, const float *__restrict__ in,
+                                                          int rows, int cols, float nv, int seg, int strips_x,
+                                                          int nitems, MeanPolicy<R> pol) {
+    typedef StripConsts<R, 4> K;
+    __shared__ __attribute__((aligned(16))) float lds[4][4][K::LDSW];
+    const int wave = threadIdx.x >> 6;
+    const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
+    if (item >= nitems) return;
+    const int sx = item % strips_x;
+    const int sy = item / strips_x;
+    const int y0 = sy * seg;
+    const int xs = sx * (K::SW - 2 * K::RP);
+    if (sy & 1)
+        strip2_march_dir<R, true, NT>(out, in, rows, cols, nv, xs, y0, min(y0 + seg, rows), pol, lds[wave]);
+    else
+        strip2_march_dir<R, false, NT>(out, in, rows, cols, nv, xs, y0, min(y0 + seg, rows), pol, lds[wave]);
+}
+
 // Row access mode of a 4-column-per-lane march over these buffers.
 template <typename T>
 inline int row_mode(const T *in, const T *out, int cols) {
@@ -137,6 +159,39 @@ static const MeanLauncher kMeanLaunchers[] = {
     launch_mean_r<10>, launch_mean_r<11>, launch_mean_r<12>, launch_mean_r<13>, launch_mean_r<14>,
     launch_mean_r<15>, launch_mean_r<16>};
 constexpr int kMaxMeanR = 16;
+
+// Two generations per launch (strip2_mean_kernel) for radii 2 .. kMaxMean2R.
+template <int R>
+int launch_mean2_r(float *out, const float *in, int rows, int cols, float nv, int seg) {
+    typedef StripConsts<R, 4> K;
+    constexpr int sw = K::SW - 2 * K::RP;  // output columns per strip
+    const bool nt = stream_stores((size_t)rows * cols * sizeof(float));
+    const long cus = ctx().num_cus;
+    const int strips_x = (cols + sw - 1) / sw;
+    if (seg <= 0) seg = tune_int("MSL_STRIP2_SEG", 0);
+    if (seg <= 0) {
+        const long target = cus * tune_int("MSL_STRIP2_WAVES", nt ? 16 : 8);
+        const long segs = (target + strips_x - 1) / strips_x;
+        seg = (int)((rows + segs - 1) / segs);
+        if (seg < 8 * R) seg = 8 * R;
+    }
+    const long nitems = (long)strips_x * ((rows + seg - 1) / seg);
+    MSL_REQUIRE(nitems <= INT32_MAX - 3, "msl_stencil2d_mean_iterate_f32: grid too large");
+    const int blocks = (int)((nitems + 3) / 4);
+    const MeanPolicy<R> pol = MeanPolicy<R>::make();
+    if (nt)
+        strip2_mean_kernel<R, true><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg, strips_x,
+                                                                     (int)nitems, pol);
+    else
+        strip2_mean_kernel<R, false><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg, strips_x,
+                                                                      (int)nitems, pol);
+    return check_launch("msl_stencil2d_mean_iterate_f32");
+}
+
+static const MeanLauncher kMean2Launchers[] = {nullptr, nullptr, launch_mean2_r<2>, launch_mean2_r<3>,
+                                               launch_mean2_r<4>, launch_mean2_r<5>, launch_mean2_r<6>,
+                                               launch_mean2_r<7>};
+constexpr int kMaxMean2R = 7;
 
 int mean_step(float *out, const float *in, int rows, int cols, int r, float nv, int seg) {
     int rc = ensure_ready();
@@ -305,6 +360,18 @@ static std::vector<int> generation_plan(int iters, int k) {
     return plan;
 }
 
+// Whether iterations of radius r run two generations per launch by default.
+// Measured (GB/s, one vs two generations per launch): radii 2-4 gain from
+// 4096^2 on (r = 2: 5.7 -> 9.0 TB/s), radii 5-7 only on grids streamed from
+// HBM (8192^2: r = 5 4.7 -> 7.2, r = 7 4.5 -> 6.1); on grids that stay in
+// the Infinity Cache their longer halo recomputation costs more than the
+// saved traffic (5792^2: r = 5 5.2 -> 4.6).
+static bool mean2_default(int r, int rows, int cols) {
+    const size_t bytes = (size_t)rows * cols * sizeof(float);
+    if (r < 2 || r > kMaxMean2R) return false;
+    return r <= 4 ? bytes >= ((size_t)32 << 20) : stream_stores(bytes);
+}
+
 static int mean3_multi_step(float *out, const float *in, int rows, int cols, float nv, int steps) {
     constexpr int TW = 64 - 2 * kMean3Steps, TH = 4 * kMean3Rows - 2 * kMean3Steps;
     const int tiles_x = (cols + TW - 1) / TW;
@@ -429,6 +496,27 @@ int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, i
                 float *dst = (k & 1) ? d_a : d_b;
                 int brc = plan[k] == 1 ? mean_step(dst, src, rows, cols, r, nv, 0)
                                        : mean3_multi_step(dst, src, rows, cols, nv, plan[k]);
+                if (brc) return brc;
+            }
+            return MSL_OK;
+        });
+    }
+    // Radii 2 .. kMaxMean2R on aligned rows: two generations per launch.
+    // MSL_MEAN_GENS = 1 or 2 forces one or two (read on every call, so a
+    // process can compare both; two needs aligned rows).
+    const char *ge = getenv("MSL_MEAN_GENS");
+    const int gens_env = ge ? atoi(ge) : 0;
+    const bool pairs = r <= kMaxMean2R && row_mode(d_a, d_b, cols) == kAligned &&
+                       (gens_env == 2 || (gens_env != 1 && mean2_default(r, rows, cols)));
+    if (pairs) {
+        key += "|2";
+        const std::vector<int> plan = generation_plan(iters, 2);
+        return run_cached_graph(key, [&]() {
+            for (size_t k = 0; k < plan.size(); k++) {
+                float *src = (k & 1) ? d_b : d_a;
+                float *dst = (k & 1) ? d_a : d_b;
+                int brc = plan[k] == 1 ? mean_step(dst, src, rows, cols, r, nv, 0)
+                                       : kMean2Launchers[r](dst, src, rows, cols, nv, 0);
                 if (brc) return brc;
             }
             return MSL_OK;

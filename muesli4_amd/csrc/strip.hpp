@@ -391,4 +391,129 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
         strip_march_dir<Pol, C, MODE, kSum, DQ, false, NT>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
 }
 
+// Two generations of the radius-R mean blur in one march (four columns per
+// lane, aligned rows).  The wave computes generation 1 on the 64 * 4 columns
+// [xs - RP, xs - RP + 256) exactly as strip_march_dir does (same staging,
+// same row sums, same division), turns every generation-1 value outside the
+// grid into nv, stages each completed generation-1 row in a second LDS row
+// and forms generation 2 from it by the same row-sum / column-sum order.
+// Lanes HL .. 63-HL, columns [xs, xs + 256 - 2RP), store generation 2; the
+// outer lanes only feed their neighbours' windows.  Input rows y0-2R ..
+// yend-1+2R are read once; each cell of both generations is computed with
+// the operation order of one generation per launch, so the result is
+// bit-identical to two single-generation marches.
+// Rows requested ahead by the two-generation march: each row carries twice
+// the work of a single-generation row, so deeper queues pay where the
+// single-generation rule prefetches one row (measured on 8192^2: r = 5
+// 5.1 -> 6.1 TB/s with 5 rows, r = 6 5.0 -> 5.8 with 2).
+constexpr int strip2_depth(int R, int d1) { return R == 5 ? 5 : (R == 6 ? 2 : d1); }
+
+template <int R, bool UP, bool NT>
+__device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const float *__restrict__ in, int rows,
+                                                 int cols, float nv, int xs, int y0, int yend,
+                                                 const MeanPolicy<R> &pol,
+                                                 float (*lds)[StripConsts<R, 4>::LDSW]) {
+    constexpr int C = 4;
+    typedef StripConsts<R, C> K;
+    constexpr int RP = K::RP;
+    constexpr int NQ = K::NQ;
+    constexpr int PER = K::PER;
+    constexpr int HL = K::HL;
+    constexpr int D = strip2_depth(R, K::D);
+    constexpr int P = ring_slots(PER, D);
+    constexpr int UL = P * D / gcd_c(P, D);
+    typedef NativeVecN<float, C>::type V;
+
+    const int lane = threadIdx.x & 63;
+    const int xg = xs - RP;         // first generation-1 column of the wave
+    const int x1 = xg + C * lane;   // this lane's generation-1 / output columns
+    const int total = (yend - y0) + 4 * R;  // input rows y0-2R .. yend-1+2R
+    const int ybase = UP ? yend - 1 + 2 * R : y0 - 2 * R;
+    constexpr int ystep = UP ? -1 : 1;
+    const bool has_halo = lane < 2 * HL;
+    const int hx = has_halo ? (lane < HL ? xg - RP + C * lane : xg + K::SW + C * (lane - HL)) : x1;
+    const int hslot = has_halo ? (lane < HL ? C * lane : RP + K::SW + C * (lane - HL)) : RP + C * lane;
+    const bool col_in = x1 >= 0 && x1 < cols;  // all four columns (cols % 4 == 0)
+    const bool store_lane = lane >= HL && lane < 64 - HL && x1 < cols;
+
+    VecN<float, C> ring1[P], ring2[P];
+    VecN<float, C> qc[D], qh[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        qc[d] = load_vec<float, C, kAligned>(in, ybase + ystep * d, x1, rows, cols);
+        qh[d] = load_vec<float, C, kAligned>(in, ybase + ystep * d, hx, rows, cols);
+    }
+
+    for (int tb = 0; tb < total; tb += UL) {
+#pragma unroll
+        for (int uu = 0; uu < UL; uu++) {
+            const int t = tb + uu;
+            const int u = uu % P;
+            const int qs = uu % D;
+            float *buf = lds[t & 1];
+            const int yy = ybase + ystep * t;
+            *reinterpret_cast<V *>(buf + RP + C * lane) =
+                to_native<float, C>(fix_vec<float, C, kAligned>(qc[qs], yy, x1, rows, cols, nv));
+            *reinterpret_cast<V *>(buf + hslot) =
+                to_native<float, C>(fix_vec<float, C, kAligned>(qh[qs], yy, hx, rows, cols, nv));
+            qc[qs] = load_vec<float, C, kAligned>(in, yy + ystep * D, x1, rows, cols);
+            qh[qs] = load_vec<float, C, kAligned>(in, yy + ystep * D, hx, rows, cols);
+            wave_lds_sync();
+            {
+                float w[C * NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const VecN<float, C> v = to_vecn<float, C>(*reinterpret_cast<const V *>(buf + C * lane + C * q));
+#pragma unroll
+                    for (int k = 0; k < C; k++) w[C * q + k] = v.v[k];
+                }
+#pragma unroll
+                for (int c = 0; c < C; c++) ring1[u].v[c] = pol.template rowsum<RP>(w, c);
+            }
+            if (t < 2 * R) continue;
+            // generation-1 row y1 is complete: finish it, stage it (nv outside the grid)
+            const int y1 = yy - ystep * R;
+            const bool ok1 = col_in && y1 >= 0 && y1 < rows;
+            VecN<float, C> v1;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                float acc = pol.colsum_init();
+#pragma unroll
+                for (int k = 0; k < PER; k++)
+                    acc = pol.acc(acc, ring1[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
+                v1.v[c] = ok1 ? pol.finish(acc, 0.0f) : nv;
+            }
+            float *gbuf = lds[2 + (t & 1)];
+            *reinterpret_cast<V *>(gbuf + RP + C * lane) = to_native<float, C>(v1);
+            wave_lds_sync();
+            {
+                float w[C * NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const VecN<float, C> v = to_vecn<float, C>(*reinterpret_cast<const V *>(gbuf + C * lane + C * q));
+#pragma unroll
+                    for (int k = 0; k < C; k++) w[C * q + k] = v.v[k];
+                }
+#pragma unroll
+                for (int c = 0; c < C; c++) ring2[u].v[c] = pol.template rowsum<RP>(w, c);
+            }
+            if (t < 4 * R) continue;
+            // output row y2: in [y0, yend) for 4R <= t < total; the unrolled
+            // steps past the end of the march (t >= total) store nothing
+            const int y2 = y1 - ystep * R;
+            VecN<float, C> o;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                float acc = pol.colsum_init();
+#pragma unroll
+                for (int k = 0; k < PER; k++)
+                    acc = pol.acc(acc, ring2[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
+                o.v[c] = pol.finish(acc, 0.0f);
+            }
+            if (store_lane && (UP ? y2 >= y0 : y2 < yend))
+                write_vec<float, C, kAligned, NT>(out + (size_t)y2 * cols, x1, cols, o);
+        }
+    }
+}
+
 }  // namespace mslhip

@@ -268,6 +268,59 @@ def test_mean_blur_r1_iterate_multi_generation(shape, iters):
         np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(rows, cols), ref)
 
 
+@pytest.fixture
+def mean_pairs(monkeypatch):
+    """Force two generations per launch for radii 2..7 (the library's default
+    on aligned grids above 64 MiB)."""
+    monkeypatch.setenv("MSL_MEAN_GENS", "2")
+
+
+# Radii 2..7 iterate two generations per launch on aligned rows: ragged
+# strips (240 or 248 output columns per wave) and segments, odd counts (one
+# single-generation launch keeps the result-buffer parity) and neutral values
+# are bit-exact against the oracle.
+@pytest.mark.parametrize("shape", [(1, 4), (5, 8), (9, 12), (37, 244), (100, 484), (70, 964), (300, 248)])
+@pytest.mark.parametrize("r", [2, 3, 5, 7])
+@pytest.mark.parametrize("iters", [2, 3, 4])
+def test_mean_blur_iterate_two_generation_launches(shape, r, iters, mean_pairs):
+    rows, cols = shape
+    n = rows * cols
+    for nv in (0.0, 3.5):
+        note(f"mean r={r} iterate pairs {shape} x{iters} nv={nv}")
+        a = dev_uniform(n, 66)
+        b = DeviceArray((rows, cols), np.float32)
+        ref = R.fill_uniform_f32(n, 66).reshape(rows, cols)
+        for _ in range(iters):
+            ref = R.stencil2d_mean_f32(ref, r, nv)
+        inb = C.c_int()
+        call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, nv, iters, C.byref(inb))
+        assert inb.value == (iters & 1)
+        np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(rows, cols), ref)
+
+
+# On large grids the pairs are the default (radii 2-4 from 32 MiB, 5-7 on
+# grids above 128 MiB): identical bits to one launch per generation, with the
+# last strip partial (4096 = 16.5 strips of 248 columns, 4104 = 17.1 of 240).
+@pytest.mark.parametrize("r,rows,cols", [(2, 4160, 4096), (4, 2100, 4096), (5, 8200, 4104), (7, 8200, 4104)])
+def test_mean_blur_iterate_large_grid_default_pairs(r, rows, cols, monkeypatch):
+    iters = 3
+    n = rows * cols
+    outs = []
+    for gens in ("1", None):
+        if gens:
+            monkeypatch.setenv("MSL_MEAN_GENS", gens)
+        else:
+            monkeypatch.delenv("MSL_MEAN_GENS", raising=False)
+        note(f"mean r={r} iterate {rows}x{cols} x{iters} gens={gens or 'default'}")
+        a = dev_uniform(n, 67)
+        b = DeviceArray((rows, cols), np.float32)
+        inb = C.c_int()
+        call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, 0.0, iters, C.byref(inb))
+        assert inb.value == 1
+        outs.append(b.to_host())
+    np.testing.assert_array_equal(outs[1], outs[0])
+
+
 def test_gol_1024_100_iterations_and_fold():
     rows = cols = 1024
     iters = 100
