@@ -314,7 +314,9 @@ __global__ __launch_bounds__(256) void mean3_multi_kernel(float *__restrict__ ou
         for (int i = 0; i < RPW; i++) {
             const float l = __int_as_float(lane_from_left(__float_as_int(c[i])));
             const float r = __int_as_float(lane_from_right(__float_as_int(c[i])));
-            rs[i + 1] = __fadd_rn(__fadd_rn(__fadd_rn(0.0f, l), c[i]), r);
+            // (no leading +0: the column sum below starts at +0, which
+            // makes a -0 row sum and a +0 one give the same result)
+            rs[i + 1] = __fadd_rn(__fadd_rn(l, c[i]), r);
         }
         edge[wave][0][lane] = rs[1];
         edge[wave][1][lane] = rs[RPW];

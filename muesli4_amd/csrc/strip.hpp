@@ -191,12 +191,16 @@ struct MeanPolicy {
         p.rcp = 1.0f / p.denom;
         return p;
     }
-    // Row sum of window w (index RP-R+c+d holds column x0+c+d-R).
+    // Row sum of window w (index RP-R+c+d holds column x0+c+d-R), dx
+    // ascending.  The definition starts the sum at +0; dropping that first
+    // add changes only a window of all -0 values (-0 instead of +0), and
+    // every row sum enters a column sum that itself starts at +0, where
+    // -0 and +0 give the same result.  So the outputs are unchanged.
     template <int RP>
     __device__ __forceinline__ float rowsum(const float *w, int c) const {
-        float s = 0.0f;
+        float s = w[RP - R + c];
 #pragma unroll
-        for (int d = 0; d <= 2 * R; d++) s = __fadd_rn(s, w[RP - R + c + d]);
+        for (int d = 1; d <= 2 * R; d++) s = __fadd_rn(s, w[RP - R + c + d]);
         return s;
     }
     // Correctly rounded colsum / denom (identical to IEEE division).

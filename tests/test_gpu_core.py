@@ -185,6 +185,33 @@ def test_mean_blur_bit_exact_small(r):
             np.testing.assert_array_equal(out.to_host(), ref, err_msg=f"r={r} shape={rows}x{cols} nv={nv}")
 
 
+# Signed zeros: the definition's sums start at +0, so a window of -0 values
+# averages to +0, while -0 neutral values and mixed zeros elsewhere must come
+# out exactly as in the oracle, through every kernel family (one generation,
+# two per launch, six per launch at r = 1).
+@pytest.mark.parametrize("r", [1, 2, 5, 10])
+@pytest.mark.parametrize("iters", [1, 2, 6])
+def test_mean_blur_signed_zeros(r, iters, monkeypatch):
+    rows, cols = 70, 296
+    rng = np.random.default_rng(17 + r)
+    x = np.full((rows, cols), -0.0, np.float32)
+    x[rng.random((rows, cols)) < 0.02] = 0.0
+    x[rng.random((rows, cols)) < 0.002] = 1.5
+    x[:, : cols // 2] = -0.0  # a half of pure -0 windows
+    for gens in ("1", "2"):
+        monkeypatch.setenv("MSL_MEAN_GENS", gens)
+        note(f"mean r={r} signed zeros x{iters} gens={gens}")
+        a = DeviceArray.from_host(x)
+        b = DeviceArray((rows, cols), np.float32)
+        ref = x.copy()
+        for _ in range(iters):
+            ref = R.stencil2d_mean_f32(ref, r, -0.0)
+        inb = C.c_int()
+        call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, -0.0, iters, C.byref(inb))
+        got = (b if inb.value else a).to_host().reshape(rows, cols)
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 def test_fast_division_matches_ieee_on_all_inputs():
     # every 32-bit pattern, every divisor (2r+1)^2 used by the blur kernels
     for r in range(1, 17):
