@@ -51,7 +51,7 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
     ap.add_argument("--kernel", default=None,
                     help="run one kernel only, --reps times, for profiler counter runs: "
-                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | blur_r1_multi | heat7 | heat7_pairs | gol | "
+                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | blur_r1_multi | blur_r5_pairs | heat7 | heat7_pairs | gol | "
                          "gol_multi | batch")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
@@ -183,8 +183,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    kname = ("mean3_multi_kernel<6, 16> (six generations per launch)" if dom_r == 1 else
-             f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true>")
+    kname = ("mean3_multi_kernel<6, 16> (six generations per launch; time per generation)" if dom_r == 1 else
+             f"strip2_mean_kernel<{dom_r}, true> (two generations per launch; time per generation)"
+             if dom_r <= 7 else f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true>")
     roofline = {"bound": "hbm",
                 "kernel": f"{kname} (8192x8192 ping-pong, 8 B/cell per iteration)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -296,6 +297,9 @@ def run_single_kernel(name, reps):
             elif name == "blur_r1_multi":  # one launch of six radius-1 generations per call
                 inb = C.c_int()
                 call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, 1, 0.0, 6, C.byref(inb))
+            elif name == "blur_r5_pairs":  # two launches of two radius-5 generations per call
+                inb = C.c_int()
+                call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, 5, 0.0, 4, C.byref(inb))
             elif name.startswith("blur_r"):
                 r = int(name[len("blur_r"):])
                 call("msl_stencil2d_mean_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, ROWS, COLS, r, 0.0)
