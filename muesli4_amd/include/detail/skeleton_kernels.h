@@ -465,11 +465,13 @@ __global__ __launch_bounds__(kThreads) void dm_stencil_kernel(T *out, const T *i
     view.neutralValue = nv;
     view.data = in;
     view.tile = ring;
-    view.tileX = xs - r;
+    const int x = xs + threadIdx.x;
+    // window origin xs - r, relative to this thread's column (in-window
+    // checks of reads within the radius then fold, see dc_stencil_kernel)
+    view.tileX = x - ((int)threadIdx.x + r);
     view.tileW = kMarchW + 2 * r;
     view.tileH = 2 * r + 1;
     view.rowMask = mask;
-    const int x = xs + threadIdx.x;
     // The callbacks capture the view by value (a per-row copy): capturing it
     // by reference miscompiles for gfx950 once a large user function is
     // inlined ("Illegal instruction detected").
@@ -520,11 +522,11 @@ __global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T
     view.border_data = border_data;
     view.border = border;
     view.tile = ring;
-    view.tileCol = xs - r;
+    const int col = xs + threadIdx.x;
+    view.tileCol = col - ((int)threadIdx.x + r);  // xs - r, relative to the thread's column
     view.tileCols = kMarchW + 2 * r;
     view.tileRows = 2 * r + 1;
     view.rowMask = mask;
-    const int col = xs + threadIdx.x;
     march2d(
         in, rows, cols, r, xs, y0, y1, ring, mask, [=](int row, int c) { return view.outside(row, c); },
         [=, &f](int row) {
@@ -633,17 +635,23 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
     const int z0 = (rest / ytiles) * zseg, z1 = min(z0 + zseg, depth);
     const int WX = kM3X + 2 * r, WY = kM3Y + 2 * r, WE = WX * WY;
     const int t = threadIdx.x;
-    // staged cells of this thread: e = t + k*256, (ex, ey) in the window
-    int ex[kM3Stage], ey[kM3Stage];
+    const size_t plane_len = (size_t)rows * cols;
+    // staged cells of this thread: e = t + k*256 in the window; their clamped
+    // in-plane offsets and x-y validity do not change from plane to plane, so
+    // a staging load is one uniform plane base plus a per-thread offset
+    size_t soff[kM3Stage];
+    bool sxy[kM3Stage];
 #pragma unroll
     for (int k = 0; k < kM3Stage; k++) {
         const int e = min(t + k * kThreads, WE - 1);
-        ex[k] = x0 - r + e % WX;
-        ey[k] = y0 - r + e / WX;
+        const int ex = x0 - r + e % WX;
+        const int ey = y0 - r + e / WX;
+        sxy[k] = (unsigned)ex < (unsigned)cols && (unsigned)ey < (unsigned)rows;
+        soff[k] = (size_t)min(max(ey, 0), rows - 1) * cols + min(max(ex, 0), cols - 1);
     }
-    auto load = [&](int x, int y, int z) -> T {
-        const int xc = min(max(x, 0), cols - 1), yc = min(max(y, 0), rows - 1), zc = min(max(z, 0), depth - 1);
-        return in[((size_t)zc * rows + yc) * cols + xc];
+    auto load = [&](int k, int z) -> T {
+        const int zc = min(max(z, 0), depth - 1);
+        return in[(size_t)zc * plane_len + soff[k]];
     };
     PLCube<T> view;
     view.width = cols;
@@ -653,19 +661,27 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
     view.neutralValue = nv;
     view.data = in;
     view.tile = ring;
-    view.tileX = x0 - r;
-    view.tileY = y0 - r;
+    const int ox = x0 + (t % kM3X), oy = y0 + t / kM3X;
+    // window origin x0 - r, y0 - r, written relative to this thread's own
+    // output cell: a read at (ox + dx, oy + dy) then has the window offset
+    // (t % kM3X) + r + dx, whose range the compiler can see, so in-window
+    // checks of reads within the radius fold away
+    view.tileX = ox - ((t % kM3X) + r);
+    view.tileY = oy - ((t / kM3X) + r);
     view.tileW = WX;
     view.tileH = WY;
     view.tileD = 2 * r + 1;
     view.planeMask = mask;
-    const int ox = x0 + (t % kM3X), oy = y0 + t / kM3X;
+    // this thread's outputs: rows oy + 4j, column ox, in-plane offsets fixed
+    size_t ooff[kM3Y / 4];
+#pragma unroll
+    for (int j = 0; j < kM3Y / 4; j++) ooff[j] = (size_t)(oy + 4 * j) * cols + ox;
     const int total = (z1 - z0) + 2 * r;
     T q[kMarch3K][kM3Stage];
 #pragma unroll
     for (int d = 0; d < kMarch3K; d++)
 #pragma unroll
-        for (int k = 0; k < kM3Stage; k++) q[d][k] = load(ex[k], ey[k], z0 - r + d);
+        for (int k = 0; k < kM3Stage; k++) q[d][k] = load(k, z0 - r + d);
     for (int sk = 0; sk < total; sk += kMarch3K) {  // uniform across the block
         const int a = z0 - r + sk;                   // first plane staged at this step
 #pragma unroll
@@ -675,10 +691,9 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
 #pragma unroll
             for (int k = 0; k < kM3Stage; k++) {
                 const int e = t + k * kThreads;
-                const bool in_cube =
-                    (unsigned)ex[k] < (unsigned)cols && (unsigned)ey[k] < (unsigned)rows && (unsigned)zz < (unsigned)depth;
+                const bool in_cube = sxy[k] && (unsigned)zz < (unsigned)depth;
                 if (e < WE) plane[e] = in_cube ? q[d][k] : nv;
-                q[d][k] = load(ex[k], ey[k], zz + kMarch3K);
+                q[d][k] = load(k, zz + kMarch3K);
             }
         }
         __syncthreads();
@@ -688,10 +703,11 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
             if (zo >= z0 && zo < z1) {
                 PLCube<T> v = view;
                 v.tileZ = zo - r;
+                T *oplane = out + (size_t)zo * plane_len;
 #pragma unroll
                 for (int j = 0; j < kM3Y / 4; j++) {
                     const int y = oy + 4 * j;
-                    if (ox < cols && y < rows) out[((size_t)zo * rows + y) * cols + ox] = f(v, ox, y, zo);
+                    if (ox < cols && y < rows) oplane[ooff[j]] = f(v, ox, y, zo);
                 }
             }
         }

@@ -156,6 +156,17 @@ MSL_USERFUNC __forceinline__ T lds_read(const T *p) {
     return *p;
 }
 
+// Element idx of an LDS array: the index is applied to the 32-bit LDS
+// pointer, so the address is 32-bit integer arithmetic (indexing the generic
+// pointer first would compute it in 64 bits).
+template <typename T>
+MSL_USERFUNC __forceinline__ T lds_read(const T *base, int idx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_arithmetic<T>::value) return ((const __attribute__((address_space(3))) T *)base)[idx];
+#endif
+    return base[idx];
+}
+
 struct Timing {
     double start = 0.0, split = 0.0;
 };

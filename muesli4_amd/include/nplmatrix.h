@@ -37,7 +37,7 @@ public:
 
     MSL_USERFUNC T operator()(int x, int y) const {
         const int lx = x - tileX, ly = y - tileY;
-        if ((unsigned)lx < (unsigned)tileW && (unsigned)ly < (unsigned)tileH) return detail::lds_read(tile + (y & rowMask) * tileW + lx);
+        if ((unsigned)lx < (unsigned)tileW && (unsigned)ly < (unsigned)tileH) return detail::lds_read(tile, (y & rowMask) * tileW + lx);
         if ((unsigned)x >= (unsigned)width || (unsigned)y >= (unsigned)height) return neutralValue;
         return data[(size_t)y * width + x];
     }

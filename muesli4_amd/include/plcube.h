@@ -38,7 +38,7 @@ public:
     MSL_USERFUNC T operator()(int x, int y, int z) const {
         const int lx = x - tileX, ly = y - tileY, lz = z - tileZ;
         if ((unsigned)lx < (unsigned)tileW && (unsigned)ly < (unsigned)tileH && (unsigned)lz < (unsigned)tileD)
-            return detail::lds_read(tile + ((z & planeMask) * tileH + ly) * tileW + lx);
+            return detail::lds_read(tile, ((z & planeMask) * tileH + ly) * tileW + lx);
         if ((unsigned)x >= (unsigned)width || (unsigned)y >= (unsigned)height || (unsigned)z >= (unsigned)depth)
             return neutralValue;
         return data[((size_t)z * height + y) * width + x];

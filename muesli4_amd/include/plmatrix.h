@@ -46,7 +46,7 @@ public:
     MSL_USERFUNC T get(int row, int col) const {
         const int lr = row - tileRow, lc = col - tileCol;
         if ((unsigned)lr < (unsigned)tileRows && (unsigned)lc < (unsigned)tileCols)
-            return detail::lds_read(tile + (row & rowMask) * tileCols + lc);
+            return detail::lds_read(tile, (row & rowMask) * tileCols + lc);
         if ((unsigned)row >= (unsigned)rows || (unsigned)col >= (unsigned)cols) return outside(row, col);
         return data[(size_t)row * cols + col];
     }
