@@ -137,7 +137,7 @@ public:
 private:
     template <msl::DCMapStencilFunctor<T> f, int RC>
     void marchStencil(DC<T> &result, int r, T nv, size_t lds) {
-        const detail::March3D m = detail::march3d_shape(ncol, nrow, depth);
+        const detail::March3D m = detail::march3d_shape(ncol, nrow, depth, lds);
         const int mask = detail::march3d_ring_planes(r) - 1;
         detail::dc_stencil_kernel<T, f, RC><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
             result.d_data, this->d_data, ncol, nrow, depth, r, nv, m.xtiles, m.ytiles, m.zseg, mask);
