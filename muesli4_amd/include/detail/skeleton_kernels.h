@@ -635,6 +635,10 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
                                                               int mask_arg) {
     const int r = RC > 0 ? RC : r_arg;
     const int mask = RC > 0 ? march3d_ring_planes(RC) - 1 : mask_arg;
+    // staged cells per thread per plane: just enough for the window plane
+    // when the radius is known (3 at r = 1), else the maximum
+    constexpr int NS = RC > 0 ? ((kM3X + 2 * RC) * (kM3Y + 2 * RC) + kThreads - 1) / kThreads : kM3Stage;
+    static_assert(NS <= kM3Stage, "window plane larger than the staging slots");
     __builtin_assume(__builtin_amdgcn_workitem_id_x() < (unsigned)kThreads);  // lets window checks fold
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *ring = reinterpret_cast<T *>(smem_raw);
@@ -648,10 +652,10 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
     // staged cells of this thread: e = t + k*256 in the window; their clamped
     // in-plane offsets and x-y validity do not change from plane to plane, so
     // a staging load is one uniform plane base plus a per-thread offset
-    size_t soff[kM3Stage];
-    bool sxy[kM3Stage];
+    size_t soff[NS];
+    bool sxy[NS];
 #pragma unroll
-    for (int k = 0; k < kM3Stage; k++) {
+    for (int k = 0; k < NS; k++) {
         const int e = min(t + k * kThreads, WE - 1);
         const int ex = x0 - r + e % WX;
         const int ey = y0 - r + e / WX;
@@ -686,11 +690,11 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
 #pragma unroll
     for (int j = 0; j < kM3Y / 4; j++) ooff[j] = (size_t)(oy + 4 * j) * cols + ox;
     const int total = (z1 - z0) + 2 * r;
-    T q[kMarch3K][kM3Stage];
+    T q[kMarch3K][NS];
 #pragma unroll
     for (int d = 0; d < kMarch3K; d++)
 #pragma unroll
-        for (int k = 0; k < kM3Stage; k++) q[d][k] = load(k, z0 - r + d);
+        for (int k = 0; k < NS; k++) q[d][k] = load(k, z0 - r + d);
     for (int sk = 0; sk < total; sk += kMarch3K) {  // uniform across the block
         const int a = z0 - r + sk;                   // first plane staged at this step
 #pragma unroll
@@ -698,7 +702,7 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
             const int zz = a + d;
             T *plane = ring + (zz & mask) * WE;
 #pragma unroll
-            for (int k = 0; k < kM3Stage; k++) {
+            for (int k = 0; k < NS; k++) {
                 const int e = t + k * kThreads;
                 const bool in_cube = sxy[k] && (unsigned)zz < (unsigned)depth;
                 if (e < WE) plane[e] = in_cube ? q[d][k] : nv;
