@@ -269,6 +269,24 @@ MSL_USERFUNC float box1_plc(const PLCube<float> &c, int x, int y, int z) {
     return v;
 }
 
+// Reads two cells away under a declared radius of 1: outside the staged
+// window the views fall back to global memory (or nv off the grid).
+MSL_USERFUNC float reach2_npl(const NPLMatrix<float> &m, int x, int y) {
+    float v = m(x - 2, y);
+    v += m(x + 2, y) * 2.0f;
+    v += m(x, y - 2) * 3.0f;
+    v += m(x, y + 2) * 4.0f;
+    v -= m(x + 1, y + 1);
+    return v;
+}
+MSL_USERFUNC float reach2_plc(const PLCube<float> &c, int x, int y, int z) {
+    float v = c(x + 2, y, z);
+    v += c(x, y - 2, z) * 2.0f;
+    v += c(x, y, z + 2) * 3.0f;
+    v += c(x - 1, y + 1, z - 1);
+    return v;
+}
+
 // ---- closed-form checks --------------------------------------------------------
 static void closed_forms() {
     const int n = 1000;
@@ -680,6 +698,9 @@ int main(int argc, char **argv) {
         write_grid(dir + "/blur_npl.bin", dst.gather(), bc, br, 1);
         src.mapStencilMeanBlur(dst2, 3, 0.5f);
         write_grid(dir + "/blur_fast.bin", dst2.gather(), bc, br, 1);
+        DM<float> reach(br, bc, 0.0f);
+        src.mapStencil<reach2_npl>(reach, 1, 0.5f);
+        write_grid(dir + "/reach_npl.bin", reach.gather(), bc, br, 1);
 
         // ---- legacy functor with a border-value functor, radius 2
         DM<float> cr(br, bc, 0.0f);
@@ -718,6 +739,9 @@ int main(int argc, char **argv) {
         c1.fill(cube.data());
         c1.mapStencil<box2_plc>(box, 2, 0.25f);
         write_grid(dir + "/box2.bin", box.gather(), nx, ny, nz);
+        DC<float> reach3(ny, nx, nz, 0.0f);
+        c1.mapStencil<reach2_plc>(reach3, 1, 0.75f);
+        write_grid(dir + "/reach_plc.bin", reach3.gather(), nx, ny, nz);
 
         // ---- float fold of the blur input through a user functor
         DA<float> flat((int)img.size());

@@ -138,6 +138,36 @@ def test_box_stencil_r2(run):
     np.testing.assert_array_equal(got, want)
 
 
+def test_stencil_reads_beyond_radius(run):
+    """Functors declared with radius 1 that read two cells away: the views
+    answer from global memory (or nv off the grid), as the reference's
+    views do for any offset (include/nplmatrix.h:123-139,
+    include/plcube.h:105-135)."""
+    f32 = np.float32
+    b = run["blur"]
+    rows, cols = b.shape
+    p = np.full((rows + 4, cols + 4), f32(0.5), f32)
+    p[2:-2, 2:-2] = b
+    at = lambda dy, dx: p[2 + dy:2 + dy + rows, 2 + dx:2 + dx + cols]
+    want = at(0, -2).copy()
+    want = want + at(0, 2) * f32(2)
+    want = want + at(-2, 0) * f32(3)
+    want = want + at(2, 0) * f32(4)
+    want = want - at(1, 1)
+    np.testing.assert_array_equal(_read(run["dir"] / "reach_npl.bin", f32), want)
+
+    h = run["heat"]
+    nz, ny, nx = h.shape
+    q = np.full((nz + 4, ny + 4, nx + 4), f32(0.75), f32)
+    q[2:-2, 2:-2, 2:-2] = h
+    at3 = lambda dz, dy, dx: q[2 + dz:2 + dz + nz, 2 + dy:2 + dy + ny, 2 + dx:2 + dx + nx]
+    want3 = at3(0, 0, 2).copy()
+    want3 = want3 + at3(0, -2, 0) * f32(2)
+    want3 = want3 + at3(2, 0, 0) * f32(3)
+    want3 = want3 + at3(-1, 1, -1)
+    np.testing.assert_array_equal(_read(run["dir"] / "reach_plc.bin", f32), want3)
+
+
 def test_user_fold_float(run):
     got = float(_read(run["dir"] / "fold_sum.bin", np.float32).ravel()[0])
     ref = math.fsum(run["blur"].astype(np.float64).ravel())
