@@ -527,7 +527,8 @@ int msl_batch_create(const msl_batch_grid *grids, int ngrids, void **h_batch) {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_step_kernel<true>, 256, 0) == hipSuccess && occ > 0)
         b->max_blocks_per_cu = occ;
-    b->blocks_per_cu = tune_int("MSL_BATCH_BLOCKS_PER_CU", b->max_blocks_per_cu);
+    b->blocks_per_cu = tune_int("MSL_BATCH_BLOCKS_PER_CU", 0);  // (cached per name: default applied below)
+    if (b->blocks_per_cu <= 0) b->blocks_per_cu = b->max_blocks_per_cu;
     b->dynamic = tune_int("MSL_BATCH_DYNAMIC", 1);
     for (int r = 0; r <= kBatchMaxR; r++)
         for (int wc = 0; wc < kWidthClasses; wc++) b->table[r][wc] = wc < 2 ? Tile{1, 64} : Tile{4, 64};

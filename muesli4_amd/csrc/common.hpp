@@ -47,7 +47,9 @@ int check_launch(const char *what);
 // Scratch workspace owned by the context (grown outside graph capture only).
 int workspace(size_t bytes, void **ptr);
 
-// Integer tuning knob from the environment (read once per name), else dflt.
+// Integer tuning knob from the environment, else dflt.  The result of the
+// first call is cached per name, dflt included: callers whose default
+// varies pass a fixed sentinel (0) and apply their default themselves.
 int tune_int(const char *name, int dflt);
 
 // Number of workgroups that fills the device for a streaming kernel.

@@ -170,7 +170,13 @@ int launch_mean2_r(float *out, const float *in, int rows, int cols, float nv, in
     const int strips_x = (cols + sw - 1) / sw;
     if (seg <= 0) seg = tune_int("MSL_STRIP2_SEG", 0);
     if (seg <= 0) {
-        const long target = cus * tune_int("MSL_STRIP2_WAVES", nt ? 16 : 8);
+        // waves per CU on streamed grids: 24 for radii 2-4, 16 above
+        // (8192^2: r = 4 7.3 -> 8.2 TB/s with 24; r = 5 7.5 with 16, 6.4 with 24);
+        // MSL_STRIP2_WAVES overrides (tune_int caches one value per name, so
+        // the radius-dependent default is applied here, not passed to it)
+        int waves = tune_int("MSL_STRIP2_WAVES", 0);
+        if (waves <= 0) waves = nt ? (R <= 4 ? 24 : 16) : 8;
+        const long target = cus * waves;
         const long segs = (target + strips_x - 1) / strips_x;
         seg = (int)((rows + segs - 1) / segs);
         if (seg < 8 * R) seg = 8 * R;
