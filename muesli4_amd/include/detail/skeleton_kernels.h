@@ -501,7 +501,9 @@ __global__ __launch_bounds__(kThreads) void dm_stencil_global_kernel(T *out, con
 }
 
 // Legacy mapStencilMM: result(row, col) = f(row, col, &view, ncol, nrow).
-template <typename T, typename R, typename F, int RC = 0>
+// kBorder: cells off the grid come from the border band (border_data) where
+// it covers them; without one they are nv.
+template <typename T, typename R, typename F, int RC = 0, bool kBorder = true>
 __global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T *in, int rows, int cols, int r_arg,
                                                                  T nv, const T *border_data, int border, int strips_x,
                                                                  int seg, int mask_arg, F f) {
@@ -528,7 +530,13 @@ __global__ __launch_bounds__(kThreads) void dm_stencil_mm_kernel(R *out, const T
     view.tileRows = 2 * r + 1;
     view.rowMask = mask;
     march2d(
-        in, rows, cols, r, xs, y0, y1, ring, mask, [=](int row, int c) { return view.outside(row, c); },
+        in, rows, cols, r, xs, y0, y1, ring, mask,
+        [=](int row, int c) {
+            if constexpr (kBorder)
+                return view.outside(row, c);
+            else
+                return nv;
+        },
         [=, &f](int row) {
             PLMatrix<T> v = view;
             v.tileRow = row - r;

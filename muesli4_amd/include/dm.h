@@ -410,9 +410,15 @@ private:
     void marchStencilMM(DM<T2> &result, F &f, int r, T nv, const T *border, int border_w, size_t lds) {
         const detail::March2D m = detail::march2d_shape(nrow, ncol);
         const int mask = detail::march2d_ring_rows(r) - 1;
-        detail::dm_stencil_mm_kernel<T, T2, F, RC><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
-            result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, m.strips_x, m.seg, mask,
-            f);
+        // without a border band the cells off the grid are nv: the staging
+        // then has no conditional load of the band
+        if (border)
+            detail::dm_stencil_mm_kernel<T, T2, F, RC, true><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+                result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, border, border_w, m.strips_x, m.seg,
+                mask, f);
+        else
+            detail::dm_stencil_mm_kernel<T, T2, F, RC, false><<<m.blocks, detail::kThreads, lds, detail::stream()>>>(
+                result.getUserFunctionData(), this->d_data, nrow, ncol, r, nv, nullptr, 0, m.strips_x, m.seg, mask, f);
     }
     template <typename In, typename F>
     void launch_index_map(const In *in, F &f) {
