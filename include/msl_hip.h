@@ -120,7 +120,8 @@ int msl_stencil2d_mean_f32(float *d_out, const float *d_in, int rows, int cols, 
 /* iters ping-pong steps between d_a and d_b starting from d_a, captured
  * once into a HIP graph and replayed.  *h_result_in_b = 1 when the final
  * grid is in d_b (odd iters).  Launches may apply several generations
- * (r = 1: up to six; r = 2..7 on large aligned grids: two) with results
+ * (r = 1: up to six; r = 2..7 on large grids with rows of 4 or more
+ * cells: two) with results
  * bit-identical to one launch per generation; d_b then holds intermediate
  * generations other than those of the one-per-launch schedule.            */
 int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, int r, float nv, int iters,

@@ -55,8 +55,8 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
 
 // Two generations of the radius-R mean blur per wave item (strip2_march_dir):
 // output strips of 256 - 2RP columns, segments marched in alternating
-// directions as in strip_stencil_kernel.  Aligned rows only.
-template <int R, bool NT>
+// directions as in strip_stencil_kernel.  Aligned or unaligned rows.
+template <int R, int MODE, bool NT>
 __global__ __launch_bounds__(256) void strip2_mean_kernel(float *__restrict__ out, const float *__restrict__ in,
                                                           int rows, int cols, float nv, int seg, int strips_x,
                                                           int nitems, MeanPolicy<R> pol) {
@@ -70,9 +70,9 @@ __global__ __launch_bounds__(256) void strip2_mean_kernel(float *__restrict__ ou
     const int y0 = sy * seg;
     const int xs = sx * (K::SW - 2 * K::RP);
     if (sy & 1)
-        strip2_march_dir<R, true, NT>(out, in, rows, cols, nv, xs, y0, min(y0 + seg, rows), pol, lds[wave]);
+        strip2_march_dir<R, MODE, true, NT>(out, in, rows, cols, nv, xs, y0, min(y0 + seg, rows), pol, lds[wave]);
     else
-        strip2_march_dir<R, false, NT>(out, in, rows, cols, nv, xs, y0, min(y0 + seg, rows), pol, lds[wave]);
+        strip2_march_dir<R, MODE, false, NT>(out, in, rows, cols, nv, xs, y0, min(y0 + seg, rows), pol, lds[wave]);
 }
 
 // Row access mode of a 4-column-per-lane march over these buffers.
@@ -185,12 +185,17 @@ int launch_mean2_r(float *out, const float *in, int rows, int cols, float nv, in
     MSL_REQUIRE(nitems <= INT32_MAX - 3, "msl_stencil2d_mean_iterate_f32: grid too large");
     const int blocks = (int)((nitems + 3) / 4);
     const MeanPolicy<R> pol = MeanPolicy<R>::make();
-    if (nt)
-        strip2_mean_kernel<R, true><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg, strips_x,
-                                                                     (int)nitems, pol);
+    const int mode = row_mode(in, out, cols);
+    MSL_REQUIRE(mode != kScalar, "msl_stencil2d_mean_iterate_f32: two generations need rows of 4 or more cells");
+    if (mode == kAligned && nt)
+        strip2_mean_kernel<R, kAligned, true><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg,
+                                                                               strips_x, (int)nitems, pol);
+    else if (mode == kAligned)
+        strip2_mean_kernel<R, kAligned, false><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg,
+                                                                                strips_x, (int)nitems, pol);
     else
-        strip2_mean_kernel<R, false><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg, strips_x,
-                                                                      (int)nitems, pol);
+        strip2_mean_kernel<R, kUnaligned, false><<<blocks, 256, 0, ctx().stream>>>(out, in, rows, cols, nv, seg,
+                                                                                  strips_x, (int)nitems, pol);
     return check_launch("msl_stencil2d_mean_iterate_f32");
 }
 
@@ -509,12 +514,12 @@ int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, i
             return MSL_OK;
         });
     }
-    // Radii 2 .. kMaxMean2R on aligned rows: two generations per launch.
-    // MSL_MEAN_GENS = 1 or 2 forces one or two (read on every call, so a
-    // process can compare both; two needs aligned rows).
+    // Radii 2 .. kMaxMean2R on rows of 4 or more cells: two generations per
+    // launch.  MSL_MEAN_GENS = 1 or 2 forces one or two (read on every call,
+    // so a process can compare both).
     const char *ge = getenv("MSL_MEAN_GENS");
     const int gens_env = ge ? atoi(ge) : 0;
-    const bool pairs = r <= kMaxMean2R && row_mode(d_a, d_b, cols) == kAligned &&
+    const bool pairs = r <= kMaxMean2R && row_mode(d_a, d_b, cols) != kScalar &&
                        (gens_env == 2 || (gens_env != 1 && mean2_default(r, rows, cols)));
     if (pairs) {
         key += "|2";

@@ -405,14 +405,15 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
 // outer lanes only feed their neighbours' windows.  Input rows y0-2R ..
 // yend-1+2R are read once; each cell of both generations is computed with
 // the operation order of one generation per launch, so the result is
-// bit-identical to two single-generation marches.
+// bit-identical to two single-generation marches.  MODE: kAligned or
+// kUnaligned rows (as in strip_march_dir).
 // Rows requested ahead by the two-generation march: each row carries twice
 // the work of a single-generation row, so deeper queues pay where the
 // single-generation rule prefetches one row (measured on 8192^2: r = 5
 // 5.1 -> 6.1 TB/s with 5 rows, r = 6 5.0 -> 5.8 with 2).
 constexpr int strip2_depth(int R, int d1) { return R == 5 ? 5 : (R == 6 ? 2 : d1); }
 
-template <int R, bool UP, bool NT>
+template <int R, int MODE, bool UP, bool NT>
 __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const float *__restrict__ in, int rows,
                                                  int cols, float nv, int xs, int y0, int yend,
                                                  const MeanPolicy<R> &pol,
@@ -437,15 +438,17 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
     const bool has_halo = lane < 2 * HL;
     const int hx = has_halo ? (lane < HL ? xg - RP + C * lane : xg + K::SW + C * (lane - HL)) : x1;
     const int hslot = has_halo ? (lane < HL ? C * lane : RP + K::SW + C * (lane - HL)) : RP + C * lane;
-    const bool col_in = x1 >= 0 && x1 < cols;  // all four columns (cols % 4 == 0)
+    bool col_in[C];  // generation-1 column inside the grid
+#pragma unroll
+    for (int c = 0; c < C; c++) col_in[c] = (unsigned)(x1 + c) < (unsigned)cols;
     const bool store_lane = lane >= HL && lane < 64 - HL && x1 < cols;
 
     VecN<float, C> ring1[P], ring2[P];
     VecN<float, C> qc[D], qh[D];
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        qc[d] = load_vec<float, C, kAligned>(in, ybase + ystep * d, x1, rows, cols);
-        qh[d] = load_vec<float, C, kAligned>(in, ybase + ystep * d, hx, rows, cols);
+        qc[d] = load_vec<float, C, MODE>(in, ybase + ystep * d, x1, rows, cols);
+        qh[d] = load_vec<float, C, MODE>(in, ybase + ystep * d, hx, rows, cols);
     }
 
     for (int tb = 0; tb < total; tb += UL) {
@@ -457,11 +460,11 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
             float *buf = lds[t & 1];
             const int yy = ybase + ystep * t;
             *reinterpret_cast<V *>(buf + RP + C * lane) =
-                to_native<float, C>(fix_vec<float, C, kAligned>(qc[qs], yy, x1, rows, cols, nv));
+                to_native<float, C>(fix_vec<float, C, MODE>(qc[qs], yy, x1, rows, cols, nv));
             *reinterpret_cast<V *>(buf + hslot) =
-                to_native<float, C>(fix_vec<float, C, kAligned>(qh[qs], yy, hx, rows, cols, nv));
-            qc[qs] = load_vec<float, C, kAligned>(in, yy + ystep * D, x1, rows, cols);
-            qh[qs] = load_vec<float, C, kAligned>(in, yy + ystep * D, hx, rows, cols);
+                to_native<float, C>(fix_vec<float, C, MODE>(qh[qs], yy, hx, rows, cols, nv));
+            qc[qs] = load_vec<float, C, MODE>(in, yy + ystep * D, x1, rows, cols);
+            qh[qs] = load_vec<float, C, MODE>(in, yy + ystep * D, hx, rows, cols);
             wave_lds_sync();
             {
                 float w[C * NQ];
@@ -477,7 +480,7 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
             if (t < 2 * R) continue;
             // generation-1 row y1 is complete: finish it, stage it (nv outside the grid)
             const int y1 = yy - ystep * R;
-            const bool ok1 = col_in && y1 >= 0 && y1 < rows;
+            const bool row1 = y1 >= 0 && y1 < rows;
             VecN<float, C> v1;
 #pragma unroll
             for (int c = 0; c < C; c++) {
@@ -485,7 +488,7 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
 #pragma unroll
                 for (int k = 0; k < PER; k++)
                     acc = pol.acc(acc, ring1[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
-                v1.v[c] = ok1 ? pol.finish(acc, 0.0f) : nv;
+                v1.v[c] = (row1 && col_in[c]) ? pol.finish(acc, 0.0f) : nv;
             }
             float *gbuf = lds[2 + (t & 1)];
             *reinterpret_cast<V *>(gbuf + RP + C * lane) = to_native<float, C>(v1);
@@ -515,7 +518,7 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
                 o.v[c] = pol.finish(acc, 0.0f);
             }
             if (store_lane && (UP ? y2 >= y0 : y2 < yend))
-                write_vec<float, C, kAligned, NT>(out + (size_t)y2 * cols, x1, cols, o);
+                write_vec<float, C, MODE, NT>(out + (size_t)y2 * cols, x1, cols, o);
         }
     }
 }

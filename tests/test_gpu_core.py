@@ -302,11 +302,12 @@ def mean_pairs(monkeypatch):
     monkeypatch.setenv("MSL_MEAN_GENS", "2")
 
 
-# Radii 2..7 iterate two generations per launch on aligned rows: ragged
-# strips (240 or 248 output columns per wave) and segments, odd counts (one
-# single-generation launch keeps the result-buffer parity) and neutral values
-# are bit-exact against the oracle.
-@pytest.mark.parametrize("shape", [(1, 4), (5, 8), (9, 12), (37, 244), (100, 484), (70, 964), (300, 248)])
+# Radii 2..7 iterate two generations per launch on rows of 4 or more cells
+# (aligned rows and odd widths): ragged strips (240 or 248 output columns per
+# wave) and segments, odd counts (one single-generation launch keeps the
+# result-buffer parity) and neutral values are bit-exact against the oracle.
+@pytest.mark.parametrize("shape", [(1, 4), (5, 8), (9, 12), (37, 244), (100, 484), (70, 964), (300, 248),
+                                   (6, 5), (11, 7), (37, 243), (100, 485), (70, 963), (3, 2)])
 @pytest.mark.parametrize("r", [2, 3, 5, 7])
 @pytest.mark.parametrize("iters", [2, 3, 4])
 def test_mean_blur_iterate_two_generation_launches(shape, r, iters, mean_pairs):
@@ -328,7 +329,8 @@ def test_mean_blur_iterate_two_generation_launches(shape, r, iters, mean_pairs):
 # On large grids the pairs are the default (radii 2-4 from 32 MiB, 5-7 on
 # grids above 128 MiB): identical bits to one launch per generation, with the
 # last strip partial (4096 = 16.5 strips of 248 columns, 4104 = 17.1 of 240).
-@pytest.mark.parametrize("r,rows,cols", [(2, 4160, 4096), (4, 2100, 4096), (5, 8200, 4104), (7, 8200, 4104)])
+@pytest.mark.parametrize("r,rows,cols", [(2, 4160, 4096), (4, 2100, 4096), (5, 8200, 4104), (7, 8200, 4104),
+                                         (3, 4160, 4095), (6, 8200, 4103)])
 def test_mean_blur_iterate_large_grid_default_pairs(r, rows, cols, monkeypatch):
     iters = 3
     n = rows * cols
