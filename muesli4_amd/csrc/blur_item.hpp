@@ -41,16 +41,16 @@ __device__ __forceinline__ void blur_item(const float *src, float *dst, int rows
                                           int y0, int y1, float *ldsw, LaneSum *ls, int up) {
     const MeanPolicy<R> pol = MeanPolicy<R>::make();
     if constexpr (R > kBatchMaxR4) {
-        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
+        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D, true, true>(
             dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls, up);
     } else if (kind == kKindAligned4) {
-        strip_march<MeanPolicy<R>, 4, kAligned, kFold, BatchDepth<R, 4>::D>(
+        strip_march<MeanPolicy<R>, 4, kAligned, kFold, BatchDepth<R, 4>::D, true, true>(
             dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls, up);
     } else if (kind == kKindUnaligned4) {
-        strip_march<MeanPolicy<R>, 4, kUnaligned, kFold, BatchDepth<R, 4>::D>(
+        strip_march<MeanPolicy<R>, 4, kUnaligned, kFold, BatchDepth<R, 4>::D, true, true>(
             dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 4>::LDSW]>(ldsw), ls, up);
     } else {
-        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D>(
+        strip_march<MeanPolicy<R>, 1, kAligned, kFold, BatchDepth<R, 1>::D, true, true>(
             dst, src, rows, cols, nv, xs, y0, y1, pol, reinterpret_cast<float(*)[StripConsts<R, 1>::LDSW]>(ldsw), ls, up);
     }
 }

@@ -283,7 +283,7 @@ constexpr int ring_slots(int PER, int D) {
 // register index is a compile-time constant.  Row t+D is requested when row t
 // is staged (DQ > 0 overrides the default prefetch depth).  With kSum, every
 // stored value is also pushed into *sum.
-template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP, bool NT>
+template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP, bool NT, bool kTail>
 __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ out,
                                                 const typename Pol::T *__restrict__ in, int rows, int cols,
                                                 typename Pol::T nv, int xs, int y0, int yend, const Pol &pol,
@@ -303,7 +303,9 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 
     const int lane = threadIdx.x & 63;
     const int x0 = xs + C * lane;
-    const int total = (yend - y0) + 2 * R;  // input rows y0-R .. yend-1+R
+    // input rows y0-R .. yend-1+R (a wave-uniform scalar when kTail ends the
+    // march at it)
+    const int total = kTail ? __builtin_amdgcn_readfirstlane((yend - y0) + 2 * R) : (yend - y0) + 2 * R;
     // input row of march step t: ascending from y0-R, or descending from yend-1+R
     const int ybase = UP ? yend - 1 + R : y0 - R;
     constexpr int ystep = UP ? -1 : 1;
@@ -327,6 +329,13 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 #pragma unroll
         for (int uu = 0; uu < UL; uu++) {
             const int t = tb + uu;
+            // kTail: the unrolled block may run past the march, and its last
+            // rows would only load, stage and sum rows that no output uses
+            // (up to a ring period per item: 6-30 % of the work of the short
+            // segments of the batch and serving engines).  Large single-grid
+            // marches lose more to the per-row exit than they save.
+            if constexpr (kTail)
+                if (t >= total) break;
             const int u = uu % P;  // ring slot of row t (tb is a multiple of P)
             const int qs = uu % D;   // queue slot of row t (tb is a multiple of D)
             T *buf = lds[t & 1];
@@ -384,15 +393,15 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 // adjacent segments makes both readers of the 2R halo rows between them
 // read those rows at the same time (both at the start or both at the end
 // of their marches), so the second read is served by L2.
-template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0, bool NT = true>
+template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0, bool NT = true, bool kTail = false>
 __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
                                             int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
                                             const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW],
                                             LaneSum *sum = nullptr, int up = 0) {
     if (up)
-        strip_march_dir<Pol, C, MODE, kSum, DQ, true, NT>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
+        strip_march_dir<Pol, C, MODE, kSum, DQ, true, NT, kTail>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
     else
-        strip_march_dir<Pol, C, MODE, kSum, DQ, false, NT>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
+        strip_march_dir<Pol, C, MODE, kSum, DQ, false, NT, kTail>(out, in, rows, cols, nv, xs, y0, yend, pol, lds, sum);
 }
 
 // Two generations of the radius-R mean blur in one march (four columns per
