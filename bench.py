@@ -183,6 +183,20 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # vector-ALU issue floor of the same kernel (rocprofv3 SQ_INSTS_VALU): the
+    # exact summation order makes the wide radii VALU-bound, not HBM-bound
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu_blur_kernels_8192.json")
+    vkey = {1: "blur_r1_multi", 5: "blur_r5_pairs", 10: "blur_r10"}.get(dom_r)
+    if vkey and os.path.exists(vpath):
+        with open(vpath) as f:
+            v = json.load(f).get(vkey)
+        if v:
+            per_gen_floor = v["issue_floor_ms"] / v["generations_per_launch"]
+            valu = {"valu_lane_instructions_per_cell": v["valu_lane_instructions_per_cell_generation"],
+                    "issue_floor_ms_per_generation": round(per_gen_floor, 5),
+                    "frac": round(per_gen_floor / launch_ms, 4),
+                    "note": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz); packed FP32 not used"}
     kname = ("mean3_multi_kernel<6, 16> (six generations per launch; time per generation)" if dom_r == 1 else
              f"strip2_mean_kernel<{dom_r}, true> (two generations per launch; time per generation)"
              if dom_r <= 7 else f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true>")
@@ -190,7 +204,7 @@ def main():
                 "kernel": f"{kname} (8192x8192 ping-pong, 8 B/cell per iteration)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": 8 * n, "avg_launch_ms": round(launch_ms, 5)}
+                "algorithmic_bytes_per_launch": 8 * n, "avg_launch_ms": round(launch_ms, 5), "valu_bound": valu}
 
     result = {
         "metric": "stencil GB/s (% HBM peak)",
