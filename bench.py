@@ -198,7 +198,7 @@ def main():
                     "frac": round(per_gen_floor / launch_ms, 4),
                     "note": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz); packed FP32 not used"}
     kname = ("mean3_multi_kernel<6, 16> (six generations per launch; time per generation)" if dom_r == 1 else
-             f"strip2_mean_kernel<{dom_r}, true> (two generations per launch; time per generation)"
+             f"strip2_mean_kernel<{dom_r}, 0, true> (two generations per launch; time per generation)"
              if dom_r <= 7 else f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true>")
     roofline = {"bound": "hbm",
                 "kernel": f"{kname} (8192x8192 ping-pong, 8 B/cell per iteration)",
