@@ -350,6 +350,25 @@ def test_mean_blur_iterate_large_grid_default_pairs(r, rows, cols, monkeypatch):
     np.testing.assert_array_equal(outs[1], outs[0])
 
 
+# Grids above 2^20 cells iterate four generations per launch (smaller ones
+# eight): both window shapes against the oracle, with ragged tiles.
+@pytest.mark.parametrize("shape,iters", [((1100, 1000), 9), ((1000, 1050), 6), ((1024, 1024), 11)])
+def test_gol_iterate_both_window_shapes(shape, iters):
+    rows, cols = shape
+    n = rows * cols
+    note(f"gol iterate {shape} x{iters}")
+    a = DeviceArray((rows, cols), np.int32)
+    b = DeviceArray((rows, cols), np.int32)
+    call("msl_fill_bernoulli_i32", a.ptr, n, 77, R.BERNOULLI_03)
+    inb = C.c_int()
+    call("msl_gol_iterate_i32", a.ptr, b.ptr, rows, cols, iters, C.byref(inb))
+    assert inb.value == (iters & 1)
+    ref = R.fill_bernoulli_i32(n, 77).reshape(rows, cols)
+    for _ in range(iters):
+        ref = R.gol_i32(ref)
+    np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(rows, cols), ref)
+
+
 def test_gol_1024_100_iterations_and_fold():
     rows = cols = 1024
     iters = 100
