@@ -126,6 +126,13 @@ int msl_stencil2d_mean_f32(float *d_out, const float *d_in, int rows, int cols, 
  * generations other than those of the one-per-launch schedule.            */
 int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, int r, float nv, int iters,
                                    int *h_result_in_b);
+/* The launch plan msl_stencil2d_mean_iterate_f32 would use for the same
+ * buffers, shape, radius and count: *h_launches launches, h_gens[k] the
+ * generations of launch k (at most `capacity` entries written).  No
+ * reference counterpart (the reference launches one generation per
+ * skeleton call); it lets callers and tests see which kernels run.      */
+int msl_stencil2d_mean_iterate_plan(const float *d_a, const float *d_b, int rows, int cols, int r, int iters,
+                                    int *h_gens, int capacity, int *h_launches);
 /* Game of Life B3/S23, cells outside are 0 (reference:
  * examples/gameoflife.cpp:18-43 through DM::mapStencilMM,
  * src/dm.cpp:615-1208).                                                 */
@@ -148,6 +155,10 @@ int msl_heat7_residual_f32(float *d_out, const float *d_in, int nx, int ny, int 
  * residual_every == 0).  Captured once into a HIP graph and replayed.   */
 int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float nv, int iters, int residual_every,
                           float *h_residuals, int *h_result_in_b);
+/* The launch plan of msl_heat7_iterate_f32 (generations per launch, in
+ * order), as msl_stencil2d_mean_iterate_plan.  No reference counterpart. */
+int msl_heat7_iterate_plan(int nx, int ny, int nz, int iters, int residual_every, int *h_gens, int capacity,
+                           int *h_launches);
 
 /* ------------------------------------------------------------------ */
 /* Stream-ordered capture of any sequence of the calls above into a

@@ -414,6 +414,26 @@ static int gol_multi_step(int32_t *out, const int32_t *in, int rows, int cols, i
     return check_launch("msl_gol_iterate_i32");
 }
 
+// Generations per launch of msl_stencil2d_mean_iterate_f32, in launch order
+// (1 = the one-step kernel; otherwise the six-generation register-window
+// kernel at r = 1 or the two-generation march at r = 2 .. kMaxMean2R on rows
+// of 4 or more cells).  MSL_MEAN_GENS = 1 or 2 forces one or two generations
+// per launch for r >= 2 (read on every call, so a process can compare both).
+static std::vector<int> mean_iterate_plan(const float *d_a, const float *d_b, int rows, int cols, int r, int iters) {
+    if (r == 1) return generation_plan(iters, kMean3Steps);
+    const char *ge = getenv("MSL_MEAN_GENS");
+    const int gens_env = ge ? atoi(ge) : 0;
+    const bool pairs = r <= kMaxMean2R && row_mode(d_a, d_b, cols) != kScalar &&
+                       (gens_env == 2 || (gens_env != 1 && mean2_default(r, rows, cols)));
+    return pairs ? generation_plan(iters, 2) : std::vector<int>(iters, 1);
+}
+
+static std::string plan_signature(const std::vector<int> &plan) {
+    std::string sig = "|";
+    for (int g : plan) sig += (char)('0' + g);
+    return sig;
+}
+
 // ---- exhaustive check of div_odd_square against IEEE division -------------
 __global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
@@ -468,6 +488,8 @@ int run_cached_graph(const std::string &key, Body body) {
         exec = (hipGraphExec_t)g;
         std::lock_guard<std::mutex> lock(g_cache.mu);
         if (g_cache.graphs.size() > 64) {
+            // a cached graph may still be running: drain the stream first
+            MSL_HIP_TRY(hipStreamSynchronize(ctx().stream));
             for (auto &kv : g_cache.graphs) (void)hipGraphExecDestroy(kv.second);
             g_cache.graphs.clear();
         }
@@ -509,50 +531,33 @@ int msl_stencil2d_mean_iterate_f32(float *d_a, float *d_b, int rows, int cols, i
     nvb.f = nv;
     MSL_REQUIRE((int64_t)rows * cols <= INT32_MAX && cols < (1 << 30), "msl_stencil2d_mean_iterate_f32: grid too large");
     MSL_REQUIRE(d_a && d_b && d_a != d_b, "msl_stencil2d_mean_iterate_f32: null or aliased buffers");
+    const std::vector<int> plan = mean_iterate_plan(d_a, d_b, rows, cols, r, iters);
     std::string key = graph_key("mean2d", d_a, d_b, rows, cols, r, nvb.i, iters);
-    if (r == 1) {  // several generations per launch
-        const std::vector<int> plan = generation_plan(iters, kMean3Steps);
-        return run_cached_graph(key, [&]() {
-            for (size_t k = 0; k < plan.size(); k++) {
-                float *src = (k & 1) ? d_b : d_a;
-                float *dst = (k & 1) ? d_a : d_b;
-                int brc = plan[k] == 1 ? mean_step(dst, src, rows, cols, r, nv, 0)
-                                       : mean3_multi_step(dst, src, rows, cols, nv, plan[k]);
-                if (brc) return brc;
-            }
-            return MSL_OK;
-        });
-    }
-    // Radii 2 .. kMaxMean2R on rows of 4 or more cells: two generations per
-    // launch.  MSL_MEAN_GENS = 1 or 2 forces one or two (read on every call,
-    // so a process can compare both).
-    const char *ge = getenv("MSL_MEAN_GENS");
-    const int gens_env = ge ? atoi(ge) : 0;
-    const bool pairs = r <= kMaxMean2R && row_mode(d_a, d_b, cols) != kScalar &&
-                       (gens_env == 2 || (gens_env != 1 && mean2_default(r, rows, cols)));
-    if (pairs) {
-        key += "|2";
-        const std::vector<int> plan = generation_plan(iters, 2);
-        return run_cached_graph(key, [&]() {
-            for (size_t k = 0; k < plan.size(); k++) {
-                float *src = (k & 1) ? d_b : d_a;
-                float *dst = (k & 1) ? d_a : d_b;
-                int brc = plan[k] == 1 ? mean_step(dst, src, rows, cols, r, nv, 0)
-                                       : kMean2Launchers[r](dst, src, rows, cols, nv, 0);
-                if (brc) return brc;
-            }
-            return MSL_OK;
-        });
-    }
+    key += plan_signature(plan);
     return run_cached_graph(key, [&]() {
-        for (int it = 0; it < iters; it++) {
-            float *src = (it & 1) ? d_b : d_a;
-            float *dst = (it & 1) ? d_a : d_b;
-            int brc = mean_step(dst, src, rows, cols, r, nv, 0);
+        for (size_t k = 0; k < plan.size(); k++) {
+            float *src = (k & 1) ? d_b : d_a;
+            float *dst = (k & 1) ? d_a : d_b;
+            int brc = plan[k] == 1   ? mean_step(dst, src, rows, cols, r, nv, 0)
+                      : r == 1       ? mean3_multi_step(dst, src, rows, cols, nv, plan[k])
+                                     : kMean2Launchers[r](dst, src, rows, cols, nv, 0);
             if (brc) return brc;
         }
         return MSL_OK;
     });
+}
+
+int msl_stencil2d_mean_iterate_plan(const float *d_a, const float *d_b, int rows, int cols, int r, int iters,
+                                    int *h_gens, int capacity, int *h_launches) {
+    MSL_REQUIRE(iters >= 0 && rows >= 0 && cols >= 0, "msl_stencil2d_mean_iterate_plan: negative argument");
+    MSL_REQUIRE(r >= 1 && r <= kMaxMeanR, "msl_stencil2d_mean_iterate_plan: radius must be in [1, 16]");
+    MSL_REQUIRE(h_launches != nullptr && capacity >= 0, "msl_stencil2d_mean_iterate_plan: bad output arguments");
+    std::vector<int> plan;
+    if (iters > 0 && rows > 0 && cols > 0) plan = mean_iterate_plan(d_a, d_b, rows, cols, r, iters);
+    *h_launches = (int)plan.size();
+    MSL_REQUIRE(h_gens != nullptr || plan.empty() || capacity == 0, "msl_stencil2d_mean_iterate_plan: null output");
+    for (size_t k = 0; k < plan.size() && (int)k < capacity; k++) h_gens[k] = plan[k];
+    return MSL_OK;
 }
 
 int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches) {

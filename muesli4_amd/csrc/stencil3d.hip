@@ -12,6 +12,7 @@
 // max |v - in| into one device scalar with an integer atomic max (the bit
 // patterns of non-negative floats order like unsigned integers).
 #include <cstdlib>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -172,6 +173,31 @@ __global__ void zero_f32_kernel(float *p, int n) {
     if (i < n) p[i] = 0.0f;
 }
 
+// Generations per launch of msl_heat7_iterate_f32, in launch order.  Two
+// generations per launch on cubes above kHeatPairMinBytes (MSL_HEAT_GENS =
+// 1 or 2 forces one or two, read on every call so a process can compare
+// both).  A residual generation must be the last of its launch, and the
+// number of launches must have the parity of iters (the result ends in b
+// exactly when iters is odd, as with one launch per generation).
+static std::vector<int> heat_iterate_plan(int nx, int ny, int nz, int iters, int residual_every) {
+    const char *ge = getenv("MSL_HEAT_GENS");
+    const int gens_env = ge ? atoi(ge) : 0;
+    const bool pairs = gens_env == 2 || (gens_env != 1 && (size_t)nx * ny * nz * sizeof(float) > kHeatPairMinBytes);
+    auto is_res = [&](int it) { return residual_every > 0 && ((it + 1) % residual_every == 0); };
+    std::vector<int> plan;
+    for (int it = 0; it < iters; it += plan.back())
+        plan.push_back((pairs && it + 1 < iters && !is_res(it)) ? 2 : 1);
+    if ((int)(plan.size() & 1) != (iters & 1)) {
+        for (size_t j = plan.size(); j-- > 0;)
+            if (plan[j] == 2) {
+                plan[j] = 1;
+                plan.insert(plan.begin() + j + 1, 1);
+                break;
+            }
+    }
+    return plan;
+}
+
 struct HeatGraph {
     hipGraphExec_t exec = nullptr;
     float *d_res = nullptr;  // residual slots owned by this graph
@@ -209,12 +235,8 @@ int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float 
     MSL_REQUIRE(nres == 0 || h_residuals != nullptr, "msl_heat7_iterate_f32: residual output required");
     if (h_result_in_b) *h_result_in_b = iters & 1;
     if (iters == 0 || nx == 0 || ny == 0 || nz == 0) return MSL_OK;
-    // generations per launch: MSL_HEAT_GENS = 1 or 2 forces (read on every
-    // call, so a process can compare both); otherwise pairs above
-    // kHeatPairMinBytes
-    const char *ge = getenv("MSL_HEAT_GENS");
-    const int gens_env = ge ? atoi(ge) : 0;
-    const bool pairs = gens_env == 2 || (gens_env != 1 && (size_t)nx * ny * nz * sizeof(float) > kHeatPairMinBytes);
+    const std::vector<int> plan = heat_iterate_plan(nx, ny, nz, iters, residual_every);
+    const bool pairs = std::find(plan.begin(), plan.end(), 2) != plan.end();
     char keybuf[256];
     snprintf(keybuf, sizeof keybuf, "heat7|%p|%p|%d|%d|%d|%08x|%d|%d|%d", (void *)d_a, (void *)d_b, nx, ny, nz,
              __builtin_bit_cast(unsigned, nv), iters, residual_every, pairs ? 2 : 1);
@@ -238,22 +260,7 @@ int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float 
             zero_f32_kernel<<<(nres + 255) / 256, 256, 0, ctx().stream>>>(d_res, nres);
             brc = check_launch("zero residuals");
         }
-        // Launches of two generations where possible: a residual generation
-        // must be the last of its launch, and the number of launches must
-        // have the parity of iters (the result ends in b exactly when iters is
-        // odd, as with one launch per generation).
         auto is_res = [&](int it) { return residual_every > 0 && ((it + 1) % residual_every == 0); };
-        std::vector<int> plan;
-        for (int it = 0; it < iters; it += plan.back())
-            plan.push_back((pairs && it + 1 < iters && !is_res(it)) ? 2 : 1);
-        if ((int)(plan.size() & 1) != (iters & 1)) {
-            for (size_t j = plan.size(); j-- > 0;)
-                if (plan[j] == 2) {
-                    plan[j] = 1;
-                    plan.insert(plan.begin() + j + 1, 1);
-                    break;
-                }
-        }
         int k = 0, it = 0;
         for (size_t j = 0; j < plan.size() && brc == MSL_OK; j++) {
             float *src = (j & 1) ? d_b : d_a;
@@ -290,6 +297,19 @@ int msl_heat7_iterate_f32(float *d_a, float *d_b, int nx, int ny, int nz, float 
                                    ctx().stream));
         MSL_HIP_TRY(hipStreamSynchronize(ctx().stream));
     }
+    return MSL_OK;
+}
+
+int msl_heat7_iterate_plan(int nx, int ny, int nz, int iters, int residual_every, int *h_gens, int capacity,
+                           int *h_launches) {
+    MSL_REQUIRE(iters >= 0 && residual_every >= 0 && nx >= 0 && ny >= 0 && nz >= 0,
+                "msl_heat7_iterate_plan: negative argument");
+    MSL_REQUIRE(h_launches != nullptr && capacity >= 0, "msl_heat7_iterate_plan: bad output arguments");
+    std::vector<int> plan;
+    if (iters > 0 && nx > 0 && ny > 0 && nz > 0) plan = heat_iterate_plan(nx, ny, nz, iters, residual_every);
+    *h_launches = (int)plan.size();
+    MSL_REQUIRE(h_gens != nullptr || plan.empty() || capacity == 0, "msl_heat7_iterate_plan: null output");
+    for (size_t k = 0; k < plan.size() && (int)k < capacity; k++) h_gens[k] = plan[k];
     return MSL_OK;
 }
 

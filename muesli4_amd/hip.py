@@ -100,11 +100,13 @@ SIGNATURES = {
     "msl_fold_max_f32_host": [p, sz, PF],
     "msl_stencil2d_mean_f32": [p, p, i32, i32, i32, f32],
     "msl_stencil2d_mean_iterate_f32": [p, p, i32, i32, i32, f32, i32, PI],
+    "msl_stencil2d_mean_iterate_plan": [p, p, i32, i32, i32, i32, PI, i32, PI],
     "msl_gol_i32": [p, p, i32, i32],
     "msl_gol_iterate_i32": [p, p, i32, i32, i32, PI],
     "msl_heat7_f32": [p, p, i32, i32, i32, f32],
     "msl_heat7_residual_f32": [p, p, i32, i32, i32, f32, p],
     "msl_heat7_iterate_f32": [p, p, i32, i32, i32, f32, i32, i32, PF, PI],
+    "msl_heat7_iterate_plan": [i32, i32, i32, i32, i32, PI, i32, PI],
     "msl_selftest_div_odd_square": [i32, C.POINTER(C.c_uint64)],
     "msl_graph_begin": [],
     "msl_graph_end": [PP],
@@ -173,6 +175,24 @@ def init(device: int = 0) -> None:
 
 def synchronize() -> None:
     call("msl_synchronize")
+
+
+def mean_iterate_plan(d_a, d_b, rows: int, cols: int, r: int, iters: int) -> list:
+    """Generations per launch that msl_stencil2d_mean_iterate_f32 would run."""
+    cap = max(iters, 1)
+    gens = (C.c_int * cap)()
+    n = C.c_int()
+    call("msl_stencil2d_mean_iterate_plan", d_a, d_b, rows, cols, r, iters, gens, cap, C.byref(n))
+    return list(gens)[: n.value]
+
+
+def heat_iterate_plan(nx: int, ny: int, nz: int, iters: int, every: int) -> list:
+    """Generations per launch that msl_heat7_iterate_f32 would run."""
+    cap = max(iters, 1)
+    gens = (C.c_int * cap)()
+    n = C.c_int()
+    call("msl_heat7_iterate_plan", nx, ny, nz, iters, every, gens, cap, C.byref(n))
+    return list(gens)[: n.value]
 
 
 def device_info() -> dict:

@@ -184,7 +184,11 @@ public:
         std::memcpy(templatepointer, localPartition, sizeof(T) * (size_t)n);
     }
 
-    T *getLocalPartition() { return localPartition; }
+    // the host mirror, refreshed from the device first (src/ds.cpp:263-267)
+    T *getLocalPartition() {
+        updateHost();
+        return localPartition;
+    }
     T *getnCPUPartition() { return localPartition; }
     void setLocalPartition(T *elements) {
         std::memcpy(localPartition, elements, sizeof(T) * (size_t)n);

@@ -167,51 +167,62 @@ float orc_max_abs_diff_f32(const float *a, const float *b, size_t n) {
 /* ------------------------------------------------------------------------ */
 /* 2-D stencils.  NPLMatrix::operator()(x, y) returns the neutral value for
  * x,y outside [0,width)x[0,height), else data[y*width + x]
- * (include/nplmatrix.h:123-143).  Loop: one iteration per flattened index k,
- * x = k % cols, y = k / cols (src/dm.cpp:1582-1589 with the GPU's x = col). */
-static inline float npl_get_f32(const float *in, int rows, int cols, float nv, int x, int y) {
-    if (x < 0 || y < 0 || x >= cols || y >= rows) return nv;
-    return in[(long)y * cols + x];
-}
-
-static inline double npl_get_f64(const double *in, int rows, int cols, double nv, int x, int y) {
-    if (x < 0 || y < 0 || x >= cols || y >= rows) return nv;
-    return in[(long)y * cols + x];
-}
-
-void orc_stencil2d_mean_f32(float *out, const float *in, int rows, int cols, int r, float nv) {
-    long n = (long)rows * cols;
-    float denom = (float)((2 * r + 1) * (2 * r + 1));
-#pragma omp parallel for schedule(static)
-    for (long k = 0; k < n; k++) {
-        int x = (int)(k % cols);
-        int y = (int)(k / cols);
-        float s = 0.0f;
-        for (int dy = -r; dy <= r; dy++) {
-            float rs = 0.0f;
-            for (int dx = -r; dx <= r; dx++) rs = rs + npl_get_f32(in, rows, cols, nv, x + dx, y + dy);
-            s = s + rs;
-        }
-        out[k] = s / denom;
+ * (include/nplmatrix.h:123-143).  The reference loop is one iteration per
+ * flattened index k, x = k % cols, y = k / cols (src/dm.cpp:1582-1589 with
+ * the GPU's x = col). */
+/* The loops below run over rows (one OpenMP iteration per row) and keep,
+ * for every cell, exactly the per-cell operation sequence of the definition:
+ * the running sums of all cells of a row are held in per-thread row buffers
+ * and advanced one (dy, dx) term at a time in ascending order, so each cell
+ * sees the same additions in the same order as a per-element loop would.
+ * Reads outside the grid take the neutral value; the bounds test is split
+ * off the interior range instead of being done per access.  Values are
+ * identical to the per-element form (checked by tests/test_oracle.py
+ * against brute-force and numpy restatements). */
+#define ORC_ROWSUM_2D(T, NAME)                                                                         \
+    void NAME(T *out, const T *in, int rows, int cols, int r, T nv) {                                   \
+        T denom = (T)((2 * r + 1) * (2 * r + 1));                                                       \
+        _Pragma("omp parallel")                                                                         \
+        {                                                                                               \
+            T *rs = (T *)malloc((size_t)cols * sizeof(T));                                              \
+            T *s = (T *)malloc((size_t)cols * sizeof(T));                                               \
+            _Pragma("omp for schedule(static)")                                                         \
+            for (int y = 0; y < rows; y++) {                                                            \
+                for (int x = 0; x < cols; x++) s[x] = (T)0;                                             \
+                for (int dy = -r; dy <= r; dy++) {                                                      \
+                    int yy = y + dy;                                                                    \
+                    for (int x = 0; x < cols; x++) rs[x] = (T)0;                                        \
+                    if (yy < 0 || yy >= rows) {                                                         \
+                        for (int dx = -r; dx <= r; dx++)                                                \
+                            for (int x = 0; x < cols; x++) rs[x] = rs[x] + nv;                          \
+                    } else {                                                                            \
+                        const T *row = in + (long)yy * cols;                                            \
+                        for (int dx = -r; dx <= r; dx++) {                                              \
+                            int lo = dx < 0 ? -dx : 0;               /* first x with x+dx >= 0 */      \
+                            int hi = dx > 0 ? cols - dx : cols;      /* end of x with x+dx < cols */   \
+                            if (lo > cols) lo = cols;                                                   \
+                            if (hi < lo) hi = lo;                                                       \
+                            for (int x = 0; x < lo; x++) rs[x] = rs[x] + nv;                            \
+                            for (int x = lo; x < hi; x++) rs[x] = rs[x] + row[x + dx];                  \
+                            for (int x = hi; x < cols; x++) rs[x] = rs[x] + nv;                         \
+                        }                                                                               \
+                    }                                                                                   \
+                    for (int x = 0; x < cols; x++) s[x] = s[x] + rs[x];                                 \
+                }                                                                                       \
+                T *o = out + (long)y * cols;                                                            \
+                for (int x = 0; x < cols; x++) o[x] = s[x] / denom;                                     \
+            }                                                                                           \
+            free(rs);                                                                                   \
+            free(s);                                                                                    \
+        }                                                                                               \
     }
-}
 
-void orc_stencil2d_mean_f64(double *out, const double *in, int rows, int cols, int r, double nv) {
-    long n = (long)rows * cols;
-    double denom = (double)((2 * r + 1) * (2 * r + 1));
-#pragma omp parallel for schedule(static)
-    for (long k = 0; k < n; k++) {
-        int x = (int)(k % cols);
-        int y = (int)(k / cols);
-        double s = 0.0;
-        for (int dy = -r; dy <= r; dy++) {
-            double rs = 0.0;
-            for (int dx = -r; dx <= r; dx++) rs += npl_get_f64(in, rows, cols, nv, x + dx, y + dy);
-            s += rs;
-        }
-        out[k] = s / denom;
-    }
-}
+/* 2-D mean blur (the reference has no 2-D mean blur; the definition is
+ * out(x,y) = (sum_dy (sum_dx in(x+dx, y+dy))) / (2r+1)^2 with both sums
+ * from 0 and ascending, evaluated like the NPLMatrix functor loop of
+ * src/dm.cpp:1582-1589 with x = col). */
+ORC_ROWSUM_2D(float, orc_stencil2d_mean_f32)
+ORC_ROWSUM_2D(double, orc_stencil2d_mean_f64)
 
 /* Game of Life functor (examples/gameoflife.cpp:18-43): sum of the eight
  * Moore neighbours read through the padded view (zero outside), then
@@ -240,75 +251,71 @@ void orc_gol_i32(int32_t *out, const int32_t *in, int rows, int cols) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* 3-D stencils.  PLCube::operator()(x, y, z) returns the neutral value
- * outside the cube, else data[(z*height + y)*width + x]
- * (include/plcube.h:105-123).  Loop over k: x = k % ncol, y = (k / ncol) %
- * nrow, z = k / (ncol*nrow) (src/dc.cpp:579-589). */
+/* heatdiffusion.cpp:55-65 per cell: v = 0; v += in(x-1); v += in(x+1);
+ * v += in(y-1); v += in(y+1); v += in(z-1); v += in(z+1); v /= 6.
+ * The loop runs over (z, y) rows; the y/z neighbour rows outside the cube
+ * point at a per-thread row of neutral values, the x neighbours outside at
+ * nv.  Each cell gets exactly the sequence above. */
+#define ORC_HEAT7_CELL(T, x, XM, XP, RESID)                                                          \
+    {                                                                                                   \
+        T v = (T)0;                                                                                     \
+        v = v + (XM);                                                                                   \
+        v = v + (XP);                                                                                   \
+        v = v + ym[x];                                                                                  \
+        v = v + yp[x];                                                                                  \
+        v = v + zm[x];                                                                                  \
+        v = v + zp[x];                                                                                  \
+        v = v / (T)6;                                                                                   \
+        o[x] = v;                                                                                       \
+        if (RESID) {                                                                                    \
+            T d = v - c[x];                                                                             \
+            d = d < 0 ? -d : d;                                                                         \
+            mres = d > mres ? d : mres;                                                                 \
+        }                                                                                               \
+    }
+
+#define ORC_HEAT7(T, NAME, RESID, RET, RETV)                                                            \
+    RET NAME(T *out, const T *in, int nx, int ny, int nz, T nv) {                                       \
+        T mres = (T)0;                                                                                  \
+        long nrows = (long)ny * nz;                                                                     \
+        _Pragma("omp parallel reduction(max : mres)")                                                   \
+        {                                                                                               \
+            T *nvrow = (T *)malloc((size_t)nx * sizeof(T));                                             \
+            for (int x = 0; x < nx; x++) nvrow[x] = nv;                                                 \
+            _Pragma("omp for schedule(static)")                                                         \
+            for (long q = 0; q < nrows; q++) {                                                          \
+                int y = (int)(q % ny), z = (int)(q / ny);                                               \
+                const T *c = in + q * nx;                                                               \
+                const T *ym = y > 0 ? c - nx : nvrow;                                                   \
+                const T *yp = y + 1 < ny ? c + nx : nvrow;                                              \
+                const T *zm = z > 0 ? c - (long)nx * ny : nvrow;                                        \
+                const T *zp = z + 1 < nz ? c + (long)nx * ny : nvrow;                                   \
+                T *o = out + q * nx;                                                                    \
+                if (nx == 1) {                                                                          \
+                    ORC_HEAT7_CELL(T, 0, nv, nv, RESID)                                                 \
+                } else {                                                                                \
+                    ORC_HEAT7_CELL(T, 0, nv, c[1], RESID)                                               \
+                    for (int x = 1; x < nx - 1; x++) ORC_HEAT7_CELL(T, x, c[x - 1], c[x + 1], RESID)    \
+                    ORC_HEAT7_CELL(T, nx - 1, c[nx - 2], nv, RESID)                                     \
+                }                                                                                       \
+            }                                                                                           \
+            free(nvrow);                                                                                \
+        }                                                                                               \
+        (void)mres;                                                                                     \
+        return RETV;                                                                                    \
+    }
+
+/* PLCube::operator()(x, y, z) returns the neutral value outside the cube,
+ * else data[(z*height + y)*width + x] (include/plcube.h:105-123); the
+ * reference loop is src/dc.cpp:579-589. */
+ORC_HEAT7(float, orc_heat7_f32, 0, void, )
+ORC_HEAT7(double, orc_heat7_f64, 0, void, )
+/* heat7 step plus max |out-in| over all cells (fused residual). */
+ORC_HEAT7(float, orc_heat7_residual_f32, 1, float, mres)
+
 static inline float plc_get_f32(const float *in, int nx, int ny, int nz, float nv, int x, int y, int z) {
     if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return nv;
     return in[((long)z * ny + y) * nx + x];
-}
-
-static inline double plc_get_f64(const double *in, int nx, int ny, int nz, double nv, int x, int y, int z) {
-    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return nv;
-    return in[((long)z * ny + y) * nx + x];
-}
-
-static inline float heat7_at(const float *in, int nx, int ny, int nz, float nv, int x, int y, int z) {
-    float v = 0.0f;
-    v = v + plc_get_f32(in, nx, ny, nz, nv, x - 1, y, z);
-    v = v + plc_get_f32(in, nx, ny, nz, nv, x + 1, y, z);
-    v = v + plc_get_f32(in, nx, ny, nz, nv, x, y - 1, z);
-    v = v + plc_get_f32(in, nx, ny, nz, nv, x, y + 1, z);
-    v = v + plc_get_f32(in, nx, ny, nz, nv, x, y, z - 1);
-    v = v + plc_get_f32(in, nx, ny, nz, nv, x, y, z + 1);
-    v = v / 6.0f;
-    return v;
-}
-
-void orc_heat7_f32(float *out, const float *in, int nx, int ny, int nz, float nv) {
-    long n = (long)nx * ny * nz;
-#pragma omp parallel for schedule(static)
-    for (long k = 0; k < n; k++) {
-        int x = (int)(k % nx);
-        int y = (int)((k / nx) % ny);
-        int z = (int)(k / ((long)nx * ny));
-        out[k] = heat7_at(in, nx, ny, nz, nv, x, y, z);
-    }
-}
-
-void orc_heat7_f64(double *out, const double *in, int nx, int ny, int nz, double nv) {
-    long n = (long)nx * ny * nz;
-#pragma omp parallel for schedule(static)
-    for (long k = 0; k < n; k++) {
-        int x = (int)(k % nx);
-        int y = (int)((k / nx) % ny);
-        int z = (int)(k / ((long)nx * ny));
-        double v = 0.0;
-        v += plc_get_f64(in, nx, ny, nz, nv, x - 1, y, z);
-        v += plc_get_f64(in, nx, ny, nz, nv, x + 1, y, z);
-        v += plc_get_f64(in, nx, ny, nz, nv, x, y - 1, z);
-        v += plc_get_f64(in, nx, ny, nz, nv, x, y + 1, z);
-        v += plc_get_f64(in, nx, ny, nz, nv, x, y, z - 1);
-        v += plc_get_f64(in, nx, ny, nz, nv, x, y, z + 1);
-        out[k] = v / 6.0;
-    }
-}
-
-float orc_heat7_residual_f32(float *out, const float *in, int nx, int ny, int nz, float nv) {
-    long n = (long)nx * ny * nz;
-    float m = 0.0f;
-#pragma omp parallel for reduction(max : m) schedule(static)
-    for (long k = 0; k < n; k++) {
-        int x = (int)(k % nx);
-        int y = (int)((k / nx) % ny);
-        int z = (int)(k / ((long)nx * ny));
-        float v = heat7_at(in, nx, ny, nz, nv, x, y, z);
-        out[k] = v;
-        float d = fabsf(v - in[k]);
-        m = d > m ? d : m;
-    }
-    return m;
 }
 
 void orc_stencil3d_boxsum_f32(float *out, const float *in, int nx, int ny, int nz, int r, float nv) {

@@ -152,51 +152,59 @@ def max_abs_diff_f32(a: np.ndarray, b: np.ndarray) -> float:
     return float(lib().orc_max_abs_diff_f32(_ptr(a), _ptr(b), a.size))
 
 
-def stencil2d_mean_f32(x: np.ndarray, r: int, nv: float = 0.0) -> np.ndarray:
+def _out(x: np.ndarray, out):
+    """A fresh output like x, or `out` (same shape and dtype, not x itself)."""
+    if out is None:
+        return np.empty_like(x)
+    assert out.shape == x.shape and out.dtype == x.dtype and out.flags.c_contiguous and out is not x
+    return out
+
+
+def stencil2d_mean_f32(x: np.ndarray, r: int, nv: float = 0.0, out=None) -> np.ndarray:
     x = np.ascontiguousarray(x, np.float32)
     rows, cols = x.shape
-    out = np.empty_like(x)
+    out = _out(x, out)
     lib().orc_stencil2d_mean_f32(_ptr(out), _ptr(x), rows, cols, r, nv)
     return out
 
 
-def stencil2d_mean_f64(x: np.ndarray, r: int, nv: float = 0.0) -> np.ndarray:
+def stencil2d_mean_f64(x: np.ndarray, r: int, nv: float = 0.0, out=None) -> np.ndarray:
     x = np.ascontiguousarray(x, np.float64)
     rows, cols = x.shape
-    out = np.empty_like(x)
+    out = _out(x, out)
     lib().orc_stencil2d_mean_f64(_ptr(out), _ptr(x), rows, cols, r, nv)
     return out
 
 
-def gol_i32(x: np.ndarray) -> np.ndarray:
+def gol_i32(x: np.ndarray, out=None) -> np.ndarray:
     x = np.ascontiguousarray(x, np.int32)
     rows, cols = x.shape
-    out = np.empty_like(x)
+    out = _out(x, out)
     lib().orc_gol_i32(_ptr(out), _ptr(x), rows, cols)
     return out
 
 
-def heat7_f32(x: np.ndarray, nv: float = 0.0) -> np.ndarray:
+def heat7_f32(x: np.ndarray, nv: float = 0.0, out=None) -> np.ndarray:
     """x has shape (nz, ny, nx)."""
     x = np.ascontiguousarray(x, np.float32)
     nz, ny, nx = x.shape
-    out = np.empty_like(x)
+    out = _out(x, out)
     lib().orc_heat7_f32(_ptr(out), _ptr(x), nx, ny, nz, nv)
     return out
 
 
-def heat7_f64(x: np.ndarray, nv: float = 0.0) -> np.ndarray:
+def heat7_f64(x: np.ndarray, nv: float = 0.0, out=None) -> np.ndarray:
     x = np.ascontiguousarray(x, np.float64)
     nz, ny, nx = x.shape
-    out = np.empty_like(x)
+    out = _out(x, out)
     lib().orc_heat7_f64(_ptr(out), _ptr(x), nx, ny, nz, nv)
     return out
 
 
-def heat7_residual_f32(x: np.ndarray, nv: float = 0.0):
+def heat7_residual_f32(x: np.ndarray, nv: float = 0.0, out=None):
     x = np.ascontiguousarray(x, np.float32)
     nz, ny, nx = x.shape
-    out = np.empty_like(x)
+    out = _out(x, out)
     m = lib().orc_heat7_residual_f32(_ptr(out), _ptr(x), nx, ny, nz, nv)
     return out, float(m)
 
@@ -211,3 +219,38 @@ def stencil3d_boxsum_f32(x: np.ndarray, r: int, nv: float = 0.0) -> np.ndarray:
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def mean_blur_iterate_f32(x: np.ndarray, r: int, iters: int, nv: float = 0.0) -> np.ndarray:
+    """`iters` oracle mean-blur steps, ping-ponging two buffers."""
+    a = np.array(x, np.float32, copy=True, order="C")
+    b = np.empty_like(a)
+    for _ in range(iters):
+        stencil2d_mean_f32(a, r, nv, out=b)
+        a, b = b, a
+    return a
+
+
+def heat7_iterate_f32(x: np.ndarray, iters: int, every: int = 0, nv: float = 0.0):
+    """`iters` oracle heat steps, a fused-residual step at every `every`-th
+    generation (as msl_heat7_iterate_f32); returns (state, residuals)."""
+    a = np.array(x, np.float32, copy=True, order="C")
+    b = np.empty_like(a)
+    res = []
+    for it in range(iters):
+        if every and (it + 1) % every == 0:
+            res.append(heat7_residual_f32(a, nv, out=b)[1])
+        else:
+            heat7_f32(a, nv, out=b)
+        a, b = b, a
+    return a, res
+
+
+def heat7_iterate_f64(x: np.ndarray, iters: int, nv: float = 0.0) -> np.ndarray:
+    """`iters` FP64 heat steps (the drift reference)."""
+    a = np.array(x, np.float64, copy=True, order="C")
+    b = np.empty_like(a)
+    for _ in range(iters):
+        heat7_f64(a, nv, out=b)
+        a, b = b, a
+    return a

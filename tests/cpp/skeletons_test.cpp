@@ -322,6 +322,14 @@ static void closed_forms() {
     ok = true;
     for (int i = 0; i < n; i++) ok &= g[i] == 5 + i;
     check(ok, "DA mapIndex");
+    // getLocalPartition refreshes the host mirror after a device skeleton
+    // (src/ds.cpp:263-267)
+    c.mapInPlace(triple);  // 3 * (i + 5)
+    int *lp = c.getLocalPartition();
+    ok = true;
+    for (int i = 0; i < n; i++) ok &= lp[i] == 3 * (5 + i);
+    check(ok, "DA getLocalPartition after mapInPlace");
+    c.mapIndex(ip, b);  // i + 5 again
     IndexMulZip iz;
     c.zipIndexInPlace(b, iz);  // i*(5+i) + 5
     g = c.gather();

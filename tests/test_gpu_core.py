@@ -19,6 +19,9 @@ pytestmark = pytest.mark.gpu
 
 # Compensated FP32 fold vs FP64 oracle: relative error bound used in tests.
 FOLD_REL_TOL = 1e-6
+# FP32 heat / blur iterations against FP64 recurrences: absolute drift bound
+# on values in [0, 1) (heat) and [0, 255) (blur, scaled by 255).
+DRIFT_ABS_TOL = 1e-5
 
 
 def note(msg):
@@ -327,27 +330,48 @@ def test_mean_blur_iterate_two_generation_launches(shape, r, iters, mean_pairs):
 
 
 # On large grids the pairs are the default (radii 2-4 from 32 MiB, 5-7 on
-# grids above 128 MiB): identical bits to one launch per generation, with the
-# last strip partial (4096 = 16.5 strips of 248 columns, 4104 = 17.1 of 240).
+# grids above 128 MiB, which store with non-temporal stores), with the last
+# strip partial (4096 = 16.5 strips of 248 columns, 4104 = 17.1 of 240).  The
+# plan query shows that two-generation launches run; results are bit-exact
+# against the oracle.
 @pytest.mark.parametrize("r,rows,cols", [(2, 4160, 4096), (4, 2100, 4096), (5, 8200, 4104), (7, 8200, 4104),
                                          (3, 4160, 4095), (6, 8200, 4103)])
 def test_mean_blur_iterate_large_grid_default_pairs(r, rows, cols, monkeypatch):
-    iters = 3
+    monkeypatch.delenv("MSL_MEAN_GENS", raising=False)
+    iters = 4
     n = rows * cols
-    outs = []
-    for gens in ("1", None):
-        if gens:
-            monkeypatch.setenv("MSL_MEAN_GENS", gens)
-        else:
-            monkeypatch.delenv("MSL_MEAN_GENS", raising=False)
-        note(f"mean r={r} iterate {rows}x{cols} x{iters} gens={gens or 'default'}")
-        a = dev_uniform(n, 67)
-        b = DeviceArray((rows, cols), np.float32)
-        inb = C.c_int()
-        call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, 0.0, iters, C.byref(inb))
-        assert inb.value == 1
-        outs.append(b.to_host())
-    np.testing.assert_array_equal(outs[1], outs[0])
+    note(f"mean r={r} iterate {rows}x{cols} x{iters} default plan")
+    a = dev_uniform(n, 67)
+    b = DeviceArray((rows, cols), np.float32)
+    plan = hip.mean_iterate_plan(a.ptr, b.ptr, rows, cols, r, iters)
+    assert plan == [2, 2], plan
+    inb = C.c_int()
+    call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, 0.0, iters, C.byref(inb))
+    assert inb.value == 0
+    ref = R.mean_blur_iterate_f32(R.fill_uniform_f32(n, 67).reshape(rows, cols), r, iters)
+    np.testing.assert_array_equal(a.to_host().reshape(rows, cols), ref)
+
+
+# The headline configuration against the oracle: 8192^2 x 20 iterations of
+# the mean blur at r = 1, 5 and 10 through the default plan (six generations
+# per launch at r = 1, two at r = 5, one at r = 10), bit for bit.
+@pytest.mark.parametrize("r,gens", [(1, 6), (5, 2), (10, 1)])
+def test_mean_blur_8192_x20_vs_oracle(r, gens, monkeypatch):
+    monkeypatch.delenv("MSL_MEAN_GENS", raising=False)
+    rows = cols = 8192
+    iters = 20
+    n = rows * cols
+    note(f"mean r={r} 8192^2 x{iters} vs oracle")
+    a = dev_uniform(n, 900 + r)
+    b = DeviceArray((rows, cols), np.float32)
+    plan = hip.mean_iterate_plan(a.ptr, b.ptr, rows, cols, r, iters)
+    assert sum(plan) == iters and max(plan) == gens and len(plan) % 2 == 0, plan
+    inb = C.c_int()
+    call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, rows, cols, r, 0.0, iters, C.byref(inb))
+    assert inb.value == 0
+    got = a.to_host().reshape(rows, cols)
+    ref = R.mean_blur_iterate_f32(R.fill_uniform_f32(n, 900 + r).reshape(rows, cols), r, iters)
+    np.testing.assert_array_equal(got, ref)
 
 
 # Grids above 2^20 cells iterate four generations per launch (smaller ones
@@ -528,28 +552,75 @@ def test_heat7_iterate_two_generation_launches(shape, iters, every, heat_pairs):
         assert list(res)[:nres] == ref_res
 
 
-def test_heat7_iterate_large_cube_default_pairs():
+def test_heat7_iterate_large_cube_default_pairs(monkeypatch):
     """A 77 MiB cube takes the paired-generation launches by default."""
+    monkeypatch.delenv("MSL_HEAT_GENS", raising=False)
     shape = (272, 272, 272)
     nz, ny, nx = shape
     iters, every = 5, 2
     note(f"heat7 iterate default {shape} x{iters}")
     n = nx * ny * nz
+    assert 2 in hip.heat_iterate_plan(nx, ny, nz, iters, every)
     a = dev_uniform(n, 46, R.SCALE_UNIT)
     b = DeviceArray(shape, np.float32)
     res = (C.c_float * (iters // every))()
     inb = C.c_int()
-    ref = R.fill_uniform_f32(n, 46, R.SCALE_UNIT).reshape(shape)
-    ref_res = []
-    for it in range(iters):
-        if (it + 1) % every == 0:
-            ref, m = R.heat7_residual_f32(ref, 0.0)
-            ref_res.append(m)
-        else:
-            ref = R.heat7_f32(ref, 0.0)
     call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+    ref, ref_res = R.heat7_iterate_f32(R.fill_uniform_f32(n, 46, R.SCALE_UNIT).reshape(shape), iters, every)
     np.testing.assert_array_equal((b if inb.value else a).to_host().reshape(shape), ref)
     assert list(res) == ref_res
+
+
+def test_heat7_iterate_pairs_above_store_threshold(monkeypatch):
+    """A 138 MiB cube: two-generation launches with non-temporal stores, a
+    fused residual ending each launch."""
+    monkeypatch.delenv("MSL_HEAT_GENS", raising=False)
+    shape = (336, 320, 336)
+    nz, ny, nx = shape
+    iters, every = 4, 2
+    note(f"heat7 iterate pairs nt {shape} x{iters}")
+    n = nx * ny * nz
+    assert hip.heat_iterate_plan(nx, ny, nz, iters, every) == [2, 2]
+    a = dev_uniform(n, 47, R.SCALE_UNIT)
+    b = DeviceArray(shape, np.float32)
+    res = (C.c_float * 2)()
+    inb = C.c_int()
+    call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+    assert inb.value == 0
+    ref, ref_res = R.heat7_iterate_f32(R.fill_uniform_f32(n, 47, R.SCALE_UNIT).reshape(shape), iters, every)
+    np.testing.assert_array_equal(a.to_host().reshape(shape), ref)
+    assert list(res) == ref_res
+
+
+# The 3-D configuration against the oracle: 512^3 x 200 heat iterations with
+# a fused residual every 10 (two generations per launch between residuals,
+# non-temporal stores), final state and all 20 residuals bit for bit; the
+# drift against an FP64 recurrence on the same inputs is reported.
+def test_heat7_512_x200_residuals_vs_oracle(monkeypatch):
+    monkeypatch.delenv("MSL_HEAT_GENS", raising=False)
+    shape = (512, 512, 512)
+    nz, ny, nx = shape
+    iters, every = 200, 10
+    n = nx * ny * nz
+    note("heat7 512^3 x200 residual every 10 vs oracle")
+    plan = hip.heat_iterate_plan(nx, ny, nz, iters, every)
+    assert sum(plan) == iters and plan.count(2) >= 80, plan
+    a = dev_uniform(n, 48, R.SCALE_UNIT)
+    b = DeviceArray(shape, np.float32)
+    res = (C.c_float * (iters // every))()
+    inb = C.c_int()
+    call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, iters, every, res, C.byref(inb))
+    assert inb.value == 0
+    got = a.to_host().reshape(shape)
+    x0 = R.fill_uniform_f32(n, 48, R.SCALE_UNIT).reshape(shape)
+    ref, ref_res = R.heat7_iterate_f32(x0, iters, every)
+    np.testing.assert_array_equal(got, ref)
+    assert list(res) == ref_res
+    del ref
+    x64 = R.heat7_iterate_f64(x0, iters)
+    d = float(np.abs(got.astype(np.float64) - x64).max())
+    print(f"[drift] heat7 512^3 after {iters} steps: max|f32-f64| = {d:.3e}")
+    assert d < DRIFT_ABS_TOL
 
 
 def test_heat7_iterate_with_residuals():
@@ -581,7 +652,6 @@ def test_heat7_iterate_with_residuals():
 # Drift of the FP32 stencils against FP64 recurrences on the same inputs
 # (reported; bounded by DRIFT_ABS_TOL).  The FP32 results themselves are
 # bit-exact against the FP32 oracle.
-DRIFT_ABS_TOL = 1e-5
 
 
 def test_heat7_drift_vs_fp64():

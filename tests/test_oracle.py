@@ -268,3 +268,83 @@ def test_heat7_f64_brute_force_and_closed_form():
     ones = np.ones((5, 5, 5))
     out = R.heat7_f64(ones, 0.0)
     assert out[2, 2, 2] == 1.0 and out[0, 0, 0] == 0.5
+
+
+# ---- numpy restatements on grids with a real interior ---------------------------
+# The oracle's stencil loops run row-wise with the bounds tests split off the
+# interior; these restatements build each cell's sum from shifted slices of a
+# padded copy (numpy float32/float64 elementwise IEEE adds in the same order),
+# so interior cells at every radius and all four borders are covered.
+def _np_mean(a, r, nv):
+    dt = a.dtype.type
+    rows, cols = a.shape
+    p = np.pad(a, r, constant_values=dt(nv))
+    s = np.zeros_like(a)
+    for dy in range(2 * r + 1):
+        rs = np.zeros_like(a)
+        for dx in range(2 * r + 1):
+            rs = rs + p[dy:dy + rows, dx:dx + cols]
+        s = s + rs
+    return s / dt((2 * r + 1) ** 2)
+
+
+def _np_heat(a, nv):
+    dt = a.dtype.type
+    p = np.pad(a, 1, constant_values=dt(nv))
+    v = np.zeros_like(a)
+    for sl in [(slice(1, -1), slice(1, -1), slice(0, -2)), (slice(1, -1), slice(1, -1), slice(2, None)),
+               (slice(1, -1), slice(0, -2), slice(1, -1)), (slice(1, -1), slice(2, None), slice(1, -1)),
+               (slice(0, -2), slice(1, -1), slice(1, -1)), (slice(2, None), slice(1, -1), slice(1, -1))]:
+        v = v + p[sl]
+    return v / dt(6)
+
+
+@pytest.mark.parametrize("r", [1, 4, 5, 10, 16])
+@pytest.mark.parametrize("shape", [(61, 97), (2, 130), (130, 3)])
+def test_mean_blur_numpy_restatement(r, shape):
+    rows, cols = shape
+    a = R.fill_uniform_f32(rows * cols, 90 + r).reshape(rows, cols)
+    a[::7, ::5] = -0.0
+    for nv in (0.0, -0.0, 2.5):
+        np.testing.assert_array_equal(R.stencil2d_mean_f32(a, r, nv).view(np.uint32),
+                                      _np_mean(a, r, nv).view(np.uint32))
+        a64 = a.astype(np.float64)
+        np.testing.assert_array_equal(R.stencil2d_mean_f64(a64, r, nv), _np_mean(a64, r, nv))
+
+
+@pytest.mark.parametrize("shape", [(23, 31, 47), (1, 9, 70), (12, 1, 5), (7, 6, 1)])
+def test_heat7_numpy_restatement(shape):
+    nz, ny, nx = shape
+    a = R.fill_uniform_f32(nz * ny * nx, 77, R.SCALE_UNIT).reshape(shape)
+    a[::3, ::2, ::5] = -0.0
+    for nv in (0.0, -0.0, 0.25):
+        expect = _np_heat(a, nv)
+        np.testing.assert_array_equal(R.heat7_f32(a, nv).view(np.uint32), expect.view(np.uint32))
+        out, m = R.heat7_residual_f32(a, nv)
+        np.testing.assert_array_equal(out.view(np.uint32), expect.view(np.uint32))
+        assert m == float(np.abs(expect - a).max())
+        a64 = a.astype(np.float64)
+        np.testing.assert_array_equal(R.heat7_f64(a64, nv), _np_heat(a64, nv))
+
+
+def test_iterate_helpers_equal_repeated_steps():
+    a = R.fill_uniform_f32(40 * 50, 5).reshape(40, 50)
+    ref = a
+    for _ in range(5):
+        ref = R.stencil2d_mean_f32(ref, 2, 1.0)
+    np.testing.assert_array_equal(R.mean_blur_iterate_f32(a, 2, 5, 1.0), ref)
+    h = R.fill_uniform_f32(6 * 7 * 8, 6, R.SCALE_UNIT).reshape(6, 7, 8)
+    ref, res = h, []
+    for it in range(7):
+        if (it + 1) % 3 == 0:
+            ref, m = R.heat7_residual_f32(ref, 0.0)
+            res.append(m)
+        else:
+            ref = R.heat7_f32(ref, 0.0)
+    got, got_res = R.heat7_iterate_f32(h, 7, 3)
+    np.testing.assert_array_equal(got, ref)
+    assert got_res == res
+    ref64 = h.astype(np.float64)
+    for _ in range(4):
+        ref64 = R.heat7_f64(ref64)
+    np.testing.assert_array_equal(R.heat7_iterate_f64(h, 4), ref64)
