@@ -103,6 +103,15 @@ int msl_mapindex_lin_f32(float *d_out, int rows, int cols);
  *   rounding.  Deterministic for a given device.                        */
 int msl_fold_sum_f32(const float *d_in, size_t n, double *d_result);
 int msl_fold_sum_f32_host(const float *d_in, size_t n, double *h_sum64, float *h_sum32);
+/* msl_fold_sum_f32 without any allocation: no reference counterpart.  For
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The block
+ * partials live in a fixed 32 KiB device array held per GPU until the
+ * process ends (at most 4096 partials; the default launch uses two per CU,
+ * and a launch shape that needs more is rejected with MSL_E_INVALID).  Same
+ * kernels and launch shape as msl_fold_sum_f32, so the result is
+ * bit-identical.  Calls that overlap on one GPU share the array and race.  */
+int msl_fold_sum_f32_noalloc(const float *d_in, size_t n, double *d_result);
 /* Wrapping uint32 sum (int fold with defined overflow). */
 int msl_fold_sum_u32_host(const uint32_t *d_in, size_t n, uint32_t *h_sum);
 /* Exact int32 sum into int64 (GoL live cells). */

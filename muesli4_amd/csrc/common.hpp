@@ -46,6 +46,10 @@ int check_launch(const char *what);
 
 // Scratch workspace owned by the context (grown outside graph capture only).
 int workspace(size_t bytes, void **ptr);
+// Looks up, with `device` current, the device address of the fixed scratch
+// array of the allocation-free fold (fold.hip); called once per device by
+// msl_init.
+int noalloc_setup(int device);
 
 // Integer tuning knob from the environment, else dflt.  The result of the
 // first call is cached per name, dflt included: callers whose default

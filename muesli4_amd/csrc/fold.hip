@@ -207,6 +207,32 @@ int run_fold(const typename P::In *in, size_t n, void *d_result, const char *nam
 
 int scratch_result(void **p) { return workspace(kResultBytes, p); }
 
+// ---- allocation-free compensated fold ---------------------------------------
+// msl_fold_sum_f32_noalloc keeps its block partials in this file-scope device
+// array instead of the context workspace, so it never allocates (callers under
+// HIP graph capture or in loops that must not call hipMalloc).  Each GPU holds
+// its own copy for the life of the process; in host code the array's name is
+// not a device address, so the address is looked up once per device with
+// hipGetSymbolAddress (noalloc_setup, called by msl_init with the device
+// current) and only that pointer is passed to kernels.  Stage 1 writes every
+// partial it later reads, so nothing needs clearing between calls.  Calls that
+// overlap on one GPU (several host threads or streams) share the array and
+// race.
+constexpr int kNoAllocMaxPartials = 4096;
+constexpr size_t kScratchBytes = (size_t)kNoAllocMaxPartials * sizeof(Pair);  // 32 KiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(MSL_E_INVALID, "msl_init: device index beyond 63");
+    if (g_scratch_ptr[device]) return MSL_OK;
+    void *p = nullptr;
+    MSL_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+    return MSL_OK;
+}
+
 }  // namespace mslhip
 
 using namespace mslhip;
@@ -216,6 +242,26 @@ extern "C" {
 int msl_fold_sum_f32(const float *d_in, size_t n, double *d_result) {
     MSL_REQUIRE(d_result != nullptr, "msl_fold_sum_f32: null result pointer");
     return run_fold<CompSumPolicy, true>(d_in, n, d_result, "msl_fold_sum_f32");
+}
+
+int msl_fold_sum_f32_noalloc(const float *d_in, size_t n, double *d_result) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    MSL_REQUIRE(d_result != nullptr, "msl_fold_sum_f32_noalloc: null result pointer");
+    MSL_REQUIRE(d_in != nullptr || n == 0, "msl_fold_sum_f32_noalloc: null input");
+    const int blocks = fold_blocks(n);
+    if (blocks > kNoAllocMaxPartials)
+        return fail(MSL_E_INVALID, "msl_fold_sum_f32_noalloc: " + std::to_string(blocks) +
+                                       " block partials exceed the 4096 the allocation-free scratch holds "
+                                       "(lower MSL_FOLD_BLOCKS_PER_CU)");
+    Pair *partials = reinterpret_cast<Pair *>(g_scratch_ptr[ctx().device]);
+    MSL_REQUIRE(partials != nullptr, "msl_fold_sum_f32_noalloc: scratch not set up on this device");
+    const int vec_ok = (((uintptr_t)d_in) & 15u) == 0;
+    fold_kernel<CompSumPolicy, true><<<blocks, kFoldThreads, 0, ctx().stream>>>(d_in, n, partials, vec_ok);
+    rc = check_launch("msl_fold_sum_f32_noalloc");
+    if (rc) return rc;
+    fold_final_kernel<CompSumPolicy, true><<<1, kFoldThreads, 0, ctx().stream>>>(partials, blocks, d_result);
+    return check_launch("msl_fold_sum_f32_noalloc");
 }
 
 int msl_fold_sum_f32_host(const float *d_in, size_t n, double *h_sum64, float *h_sum32) {

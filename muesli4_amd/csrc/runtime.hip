@@ -102,6 +102,9 @@ int msl_init(int device) {
     MSL_HIP_TRY(hipSetDevice(device));
     hipDeviceProp_t prop;
     MSL_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    // the allocation-free fold's scratch address on this device
+    int rc = noalloc_setup(device);
+    if (rc) return rc;
     g_ctx.device = device;
     g_ctx.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     // Default-priority, non-blocking stream (one per device as in the reference).

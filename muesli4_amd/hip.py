@@ -95,6 +95,7 @@ SIGNATURES = {
     "msl_mapindex_lin_f32": [p, i32, i32],
     "msl_fold_sum_f32": [p, sz, p],
     "msl_fold_sum_f32_host": [p, sz, PD, PF],
+    "msl_fold_sum_f32_noalloc": [p, sz, p],
     "msl_fold_sum_u32_host": [p, sz, PU32],
     "msl_fold_sum_i32_host": [p, sz, PI64],
     "msl_fold_max_f32_host": [p, sz, PF],
