@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s (1e9 B/s)
+# wave64 FP32 VALU instructions per second: 1024 SIMDs x 2.4 GHz / 4 cycles
+# each (packed FP32 instructions are not used)
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 ROWS = COLS = 8192
 RADII = (1, 5, 10)
 BLUR_ITERS = 20
@@ -42,6 +45,49 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _profile_json(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def radius_figures(hip, a, b, r, t_ms):
+    """Figures of one radius of the headline: the event-timed 20-iteration
+    call (t_ms), its launch plan, the HBM bytes the rocprofv3 counters saw
+    for one such call and its vector-ALU issue floor (SQ_INSTS_VALU)."""
+    n = ROWS * COLS
+    plan = hip.mean_iterate_plan(a.ptr, b.ptr, ROWS, COLS, r, BLUR_ITERS)
+    gens = max(plan)
+    kname = ("mean3_multi_kernel<6, 16>" if r == 1 and gens > 1 else
+             f"strip2_mean_kernel<{r}, 0, true>" if gens == 2 else
+             f"strip_stencil_kernel<MeanPolicy<{r}>, 0, true, 4>")
+    out = {"kernel": kname, "plan_generations_per_launch": plan, "plan": plan,
+           "ms_per_iteration": round(t_ms / BLUR_ITERS, 5),
+           "effective_GBs": round(8 * n * BLUR_ITERS / (t_ms * 1e-3) / 1e9, 1),
+           "hbm_bytes_per_iteration": None}
+    tr = _profile_json(f"traffic_iterate_r{r}_8192.json")
+    hbm_frac = None
+    if tr:
+        per_call = tr["hbm_bytes_per_call"]
+        gbs = per_call / (t_ms * 1e-3) / 1e9
+        hbm_frac = gbs / HBM_PEAK_GBS
+        out.update({"hbm_bytes_per_iteration": per_call // BLUR_ITERS,
+                    "hbm_bytes_per_launch": per_call // len(plan),
+                    "achieved_hbm_GBs": round(gbs, 1), "pct_hbm_peak_achieved": round(100 * hbm_frac, 1)})
+    va = _profile_json(f"valu_iterate_r{r}_8192.json")
+    if va:
+        floor_ms = va["valu_wave_instructions_per_call"] / VALU_ISSUE_PER_S * 1e3
+        out.update({"valu_lane_instructions_per_cell_generation": va["valu_lane_instructions_per_cell_generation"],
+                    "valu_floor_ms_per_iteration": round(floor_ms / BLUR_ITERS, 5),
+                    "valu_floor_ms_per_launch": round(floor_ms / len(plan), 5),
+                    "valu_frac": round(floor_ms / t_ms, 4)})
+    if hbm_frac is not None and "valu_frac" in out:
+        out["bound"] = "valu" if out["valu_frac"] > hbm_frac else "hbm"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,8 +97,8 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="skip the sub-workloads")
     ap.add_argument("--kernel", default=None,
                     help="run one kernel only, --reps times, for profiler counter runs: "
-                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | blur_r1_multi | blur_r5_pairs | heat7 | heat7_pairs | gol | "
-                         "gol_multi | batch")
+                         "map | zip | fold | blur_r1 | blur_r5 | blur_r10 | iter_r1 | iter_r5 | iter_r10 | "
+                         "blur_r1_multi | blur_r5_pairs | heat7 | heat7_pairs | heat_iter | gol | gol_multi | batch")
     ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     if args.kernel:
@@ -169,42 +215,40 @@ def main():
     med = {r: float(np.median(v)) for r, v in blur_ms.items()}
     blur_bytes = 8 * n * BLUR_ITERS * len(RADII)
     value = blur_bytes / (sum(med.values()) * 1e-3) / 1e9
-    per_radius = {f"r{r}": {"ms_per_iteration": round(t / BLUR_ITERS, 5),
-                            "GBs": round(8 * n * BLUR_ITERS / (t * 1e-3) / 1e9, 1)} for r, t in med.items()}
-    log(f"[bench] headline blur GB/s {value:.1f} per radius {per_radius} step ms {np.median(step_ms):.3f}")
+    per_radius = {f"r{r}": radius_figures(hip, a, b, r, t) for r, t in med.items()}
+    hbm_total = [per_radius[f"r{r}"]["hbm_bytes_per_iteration"] for r in RADII]
+    hbm_achieved = None
+    if all(x is not None for x in hbm_total):
+        gbs = BLUR_ITERS * sum(hbm_total) / (sum(med.values()) * 1e-3) / 1e9
+        hbm_achieved = {"GBs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
+                        "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE bytes of one 20-iteration call per radius "
+                                  "(profiles/traffic_iterate_r*_8192.json) over the event-timed iterations"}
+    log(f"[bench] headline blur effective GB/s {value:.1f} achieved {hbm_achieved} per radius {per_radius} "
+        f"step ms {np.median(step_ms):.3f}")
 
     # ---- roofline of the dominant kernel: the radius with the largest share
-    # of the timed region; average launch duration = its 20-step time / 20.
+    # of the timed region; average launch duration = its 20-step time divided
+    # by the launches of its plan.
     dom_r = max(med, key=lambda r: med[r])
-    launch_ms = med[dom_r] / BLUR_ITERS
-    achieved = 8 * n / (launch_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_blur_r{dom_r}_8192.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    # vector-ALU issue floor of the same kernel (rocprofv3 SQ_INSTS_VALU): the
-    # exact summation order makes the wide radii VALU-bound, not HBM-bound
-    valu = None
-    vpath = os.path.join(ROOT, "profiles", "valu_blur_kernels_8192.json")
-    vkey = {1: "blur_r1_multi", 5: "blur_r5_pairs", 10: "blur_r10"}.get(dom_r)
-    if vkey and os.path.exists(vpath):
-        with open(vpath) as f:
-            v = json.load(f).get(vkey)
-        if v:
-            per_gen_floor = v["issue_floor_ms"] / v["generations_per_launch"]
-            valu = {"valu_lane_instructions_per_cell": v["valu_lane_instructions_per_cell_generation"],
-                    "issue_floor_ms_per_generation": round(per_gen_floor, 5),
-                    "frac": round(per_gen_floor / launch_ms, 4),
-                    "note": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz); packed FP32 not used"}
-    kname = ("mean3_multi_kernel<6, 16> (six generations per launch; time per generation)" if dom_r == 1 else
-             f"strip2_mean_kernel<{dom_r}, 0, true> (two generations per launch; time per generation)"
-             if dom_r <= 7 else f"strip_stencil_kernel<MeanPolicy<{dom_r}>, 0, true>")
-    roofline = {"bound": "hbm",
-                "kernel": f"{kname} (8192x8192 ping-pong, 8 B/cell per iteration)",
+    pr = per_radius[f"r{dom_r}"]
+    gens = max(pr["plan"])
+    launch_ms = med[dom_r] / len(pr["plan"]) if gens == 1 else pr["ms_per_iteration"] * gens
+    alg_per_launch = 8 * n * gens
+    achieved = alg_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = pr.get("hbm_bytes_per_launch")
+    hbm_frac = (traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else achieved / HBM_PEAK_GBS
+    valu_frac = pr.get("valu_frac")
+    bound = "valu" if (valu_frac is not None and valu_frac > hbm_frac) else "hbm"
+    roofline = {"bound": bound,
+                "kernel": f"{pr['kernel']} (8192x8192 ping-pong, 8 B/cell per generation)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": 8 * n, "avg_launch_ms": round(launch_ms, 5), "valu_bound": valu}
+                "algorithmic_bytes_per_launch": alg_per_launch, "avg_launch_ms": round(launch_ms, 5),
+                "hbm_achieved_GBs": round(hbm_frac * HBM_PEAK_GBS, 1), "hbm_frac": round(hbm_frac, 4),
+                "valu_frac": valu_frac, "valu_floor_ms_per_launch": pr.get("valu_floor_ms_per_launch"),
+                "note": "achieved/frac: algorithmic bytes per launch over the event-timed launch; "
+                        "hbm_achieved/hbm_frac: rocprofv3 FETCH_SIZE+WRITE_SIZE bytes over the same time; "
+                        "valu_frac: SQ_INSTS_VALU issue floor over the same time (bound = the larger)"}
 
     result = {
         "metric": "stencil GB/s (% HBM peak)",
@@ -222,7 +266,11 @@ def main():
         "config": {"workload": "DM<float> 8192x8192 mean-blur mapStencil r=1,5,10 x 20 iterations each "
                                "(+ map x*1.5f+3.f, zip a+b, mapIndex row*cols+col, compensated fold sum)",
                    "rows": ROWS, "cols": COLS, "radii": list(RADII), "iterations": BLUR_ITERS},
-        "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 1),
+        "value_kind": "effective: 8 B per cell per generation (one read, one write), so launches that fuse "
+                      "generations (six at r=1, two at r=5) count more bytes than they move; hbm_achieved "
+                      "gives the bytes the rocprofv3 counters see",
+        "pct_hbm_peak_effective": round(100 * value / HBM_PEAK_GBS, 1),
+        "hbm_achieved": hbm_achieved,
         "roofline": roofline,
         "per_radius": per_radius,
         "kernels": kernels,
@@ -232,17 +280,17 @@ def main():
         "device": info,
     }
 
+    extra = {}  # GPU outputs that the CPU legs compare with (final states, sums)
     if not args.only_headline:
         result["gol"] = bench_gol(hip, call, DeviceArray)
-        result["heat3d"] = bench_heat(hip, call, DeviceArray)
-        result["batched"] = bench_batch(hip)
-        result["serving"] = bench_serve(hip)
+        result["heat3d"] = bench_heat(hip, call, DeviceArray, extra)
+        result["batched"] = bench_batch(hip, extra)
+        result["serving"] = bench_serve(hip, extra)
         result["cpp_api"] = bench_cpp_api()
 
+    result["cpu_baseline"] = None
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(value)
-    else:
-        result["cpu_baseline"] = None
+        cpu_baselines(result, value, kernel_ms, extra)
     print(json.dumps(result), flush=True)
 
 
@@ -265,7 +313,7 @@ def run_single_kernel(name, reps):
         hip.synchronize()
         log(f"[bench] ran batch x{max(1, reps // BATCH_ITERS)} runs of {BATCH_ITERS} steps")
         return
-    if name in ("heat7", "heat7_pairs"):
+    if name in ("heat7", "heat7_pairs", "heat_iter"):
         nx = ny = nz = 512
         n = nx * ny * nz
         a = DeviceArray(n, np.float32)
@@ -276,6 +324,9 @@ def run_single_kernel(name, reps):
         for i in range(reps + 2):
             if name == "heat7":
                 call("msl_heat7_f32", bufs[(i + 1) % 2].ptr, bufs[i % 2].ptr, nx, ny, nz, 0.0)
+            elif name == "heat_iter":  # the 200-iteration run with a residual every 10 (the heat3d graph)
+                res = (C.c_float * 20)()
+                call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, 200, 10, res, C.byref(inb))
             else:  # two launches of two generations each per call (four keeps the buffer parity)
                 call("msl_heat7_iterate_f32", a.ptr, b.ptr, nx, ny, nz, 0.0, 4, 0, None, C.byref(inb))
     elif name in ("gol", "gol_multi"):
@@ -308,6 +359,10 @@ def run_single_kernel(name, reps):
                 call("msl_zip_add_f32", c.ptr, a.ptr, b.ptr, n)
             elif name == "fold":
                 call("msl_fold_sum_f32", a.ptr, n, d_res.ptr)
+            elif name.startswith("iter_r"):  # one 20-iteration call (the headline's graph) per rep
+                inb = C.c_int()
+                r = int(name[len("iter_r"):])
+                call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, r, 0.0, BLUR_ITERS, C.byref(inb))
             elif name == "blur_r1_multi":  # one launch of six radius-1 generations per call
                 inb = C.c_int()
                 call("msl_stencil2d_mean_iterate_f32", a.ptr, b.ptr, ROWS, COLS, 1, 0.0, 6, C.byref(inb))
@@ -352,7 +407,7 @@ def bench_gol(hip, call, DeviceArray):
     return out
 
 
-def bench_heat(hip, call, DeviceArray):
+def bench_heat(hip, call, DeviceArray, extra):
     nx = ny = nz = 512
     iters, every = 200, 10
     n = nx * ny * nz
@@ -378,11 +433,12 @@ def bench_heat(hip, call, DeviceArray):
                        "fused max|delta| residual every 10, hipGraph loop",
            "ms": round(t * 1e3, 3), "GBs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
            "residuals": [float(x) for x in res]}
+    extra["heat_final"] = (b if inb.value else a).to_host()
     log(f"[bench] heat {out['GBs']} GB/s")
     return out
 
 
-def bench_batch(hip):
+def bench_batch(hip, extra):
     """4096 independent grids (long side log-uniform in [32,1024], aspect up
     to 4:1, r uniform in 1..8), 10 mean-blur steps each plus a per-grid
     fold(sum): persistent launches in one graph vs the plain loop of one
@@ -426,6 +482,9 @@ def bench_batch(hip):
            "launches_loop": int(BATCH_GRIDS * (BATCH_ITERS + 1)), "launches_persistent_graph": BATCH_ITERS + 1,
            "autotune_s": round(tune_s, 2), "autotuned_classes": tuned, "launch": info,
            "tiles_cols_per_lane_seg_by_width_class": tiles}
+    bt.reset()
+    bt.run(BATCH_ITERS)
+    extra["batch_sums"] = bt.fold_sums()
     log(f"[bench] batched {out}")
     bt.close()
     return out
@@ -447,7 +506,7 @@ def bench_cpp_api():
     return out
 
 
-def bench_serve(hip):
+def bench_serve(hip, extra):
     """Serving: a seeded mix of 2-D mean-blur jobs (sides log-uniform in
     [64,2048], r 1..10, 10..500 steps), 3-D heat jobs (side 32..256, 10..100
     steps) and u32 map+fold jobs (length log-uniform in [1e4,1e8]).
@@ -494,14 +553,55 @@ def bench_serve(hip):
     return out
 
 
-def cpu_baseline(gpu_value):
-    """Oracle (OpenMP C) on the host cores, same metric and mix as the GPU
-    headline: full 8192x8192 grid, one mean-blur iteration at each of r=1, 5,
-    10 per round, rounds repeated until about 10 s have elapsed (at most 20)."""
+# ---- CPU baseline legs -------------------------------------------------------
+# The oracle (OpenMP C, test infrastructure) timed on this host's cores beside
+# every GPU figure: rank 0, one GPU, bounded samples of the same workloads.
+def _cpu_time(fn, reps=5, budget_s=5.0):
+    """Median wall time of fn() over up to `reps` runs (one warm-up), stopping
+    once `budget_s` seconds have been spent."""
+    fn()
+    ts = []
+    t_start = time.perf_counter()
+    while len(ts) < reps and (not ts or time.perf_counter() - t_start < budget_s):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), len(ts)
+
+
+def cpu_baselines(result, gpu_value, kernel_ms, extra):
     from oracle import refcpu as R
     threads = R.num_threads()
+    log(f"[bench] cpu legs on {threads} OpenMP threads")
+    result["cpu_baseline"] = cpu_headline(R, threads, gpu_value)
+    result["cpu_baseline_kernels"] = cpu_kernels(R, threads, kernel_ms)
+    if "gol" in result:
+        result["gol"]["cpu_baseline"] = cpu_gol(R, threads, result["gol"])
+    if "heat3d" in result:
+        result["heat3d"]["cpu_baseline"] = cpu_heat(R, threads, result["heat3d"], extra)
+    if "batched" in result:
+        result["batched"]["cpu_baseline"] = cpu_batch(R, threads, result["batched"], extra)
+    if "serving" in result:
+        result["serving"]["cpu_baseline"] = cpu_serve(R, threads, result["serving"], extra)
+    if "cpp_api" in result and isinstance(result["cpp_api"], dict):
+        kb = result["cpu_baseline_kernels"]
+        pr = result["cpu_baseline"]["per_radius_GBs"]
+        result["cpp_api"]["cpu_baseline"] = {
+            "unit": "GB/s (algorithmic) or us/step", "cores": threads, "kind": "port",
+            "map": kb["map"]["GBs"], "zip": kb["zip"]["GBs"], "fold": kb["fold"]["GBs"],
+            "blur_r1_one_step": pr["r1"],
+            "heat7_512_one_step": result.get("heat3d", {}).get("cpu_baseline", {}).get("GBs"),
+            "gol_1024_us_per_step": result.get("gol", {}).get("cpu_baseline", {}).get("us_per_step"),
+            "sample": "the same workloads timed in the other CPU legs (oracle, same sizes)"}
+
+
+def cpu_headline(R, threads, gpu_value):
+    """Same metric and mix as the GPU headline: full 8192x8192 grid, one
+    mean-blur iteration at each of r=1, 5, 10 per round, rounds repeated until
+    about 10 s have elapsed (at most 20)."""
     n = ROWS * COLS
     x = R.fill_uniform_f32(n, SEED).reshape(ROWS, COLS)
+    y = np.empty_like(x)
     R.stencil2d_mean_f32(x[:16], 1, 0.0)  # warm the thread pool
     per_t = {r: [] for r in RADII}
     rounds = 0
@@ -509,20 +609,159 @@ def cpu_baseline(gpu_value):
     while rounds < 20 and (rounds == 0 or total_t < 10.0):
         for r in RADII:
             t0 = time.perf_counter()
-            R.stencil2d_mean_f32(x, r, 0.0)
+            R.stencil2d_mean_f32(x, r, 0.0, out=y)
             dt = time.perf_counter() - t0
             per_t[r].append(dt)
             total_t += dt
-            log(f"[bench] cpu r={r} {dt:.2f}s")
         rounds += 1
     med = {r: float(np.median(v)) for r, v in per_t.items()}
     v = 8 * n * len(RADII) / sum(med.values()) / 1e9
+    log(f"[bench] cpu headline {v:.2f} GB/s")
     return {"value": round(v, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"oracle mean-blur on the full 8192x8192 grid, one iteration at each of r=1,5,10 per round, "
                       f"{rounds} round(s), {total_t:.1f} s of CPU time; value = 3 x 8 B/cell / sum of per-radius "
                       "median iteration times (the GPU headline's equal-iterations mix)",
             "per_radius_GBs": {f"r{r}": round(8 * n / t / 1e9, 3) for r, t in med.items()},
             "gpu_over_cpu": round(gpu_value / v, 1) if v > 0 else None}
+
+
+def cpu_kernels(R, threads, kernel_ms):
+    """map / zip / mapIndex / fold on the 8192x8192 grid, each the median of
+    up to 5 runs."""
+    n = ROWS * COLS
+    x = R.fill_uniform_f32(n, SEED)
+    y = R.fill_uniform_f32(n, SEED + 1)
+    lib = R.lib()
+    out = np.empty_like(x)
+    legs = {
+        "map": (lambda: lib.orc_map_affine_f32(out.ctypes.data, x.ctypes.data, n, 1.5, 3.0), 8 * n),
+        "zip": (lambda: lib.orc_zip_add_f32(out.ctypes.data, x.ctypes.data, y.ctypes.data, n), 12 * n),
+        "mapindex": (lambda: lib.orc_mapindex_lin_f32(out.ctypes.data, ROWS, COLS), 4 * n),
+        "fold": (lambda: lib.orc_fold_sum_f64(x.ctypes.data, n), 4 * n),
+    }
+    res = {}
+    for k, (fn, nbytes) in legs.items():
+        t, reps = _cpu_time(fn)
+        gbs = nbytes / t / 1e9
+        gpu_gbs = nbytes / (kernel_ms[k] * 1e-3) / 1e9
+        res[k] = {"ms": round(t * 1e3, 3), "GBs": round(gbs, 2), "runs": reps, "gpu_over_cpu": round(gpu_gbs / gbs, 1)}
+    res["cores"] = threads
+    res["kind"] = "port"
+    res["sample"] = "oracle loops on the full 8192x8192 grid (fold: the FP64 oracle sum), median of up to 5 runs"
+    log(f"[bench] cpu kernels {res}")
+    return res
+
+
+def cpu_gol(R, threads, gpu):
+    rows = cols = 1024
+    iters = 100
+    n = rows * cols
+    x0 = R.fill_bernoulli_i32(n, SEED).reshape(rows, cols)
+    bufs = [x0.copy(), np.empty_like(x0)]
+
+    def run():
+        bufs[0][...] = x0
+        for i in range(iters):
+            R.gol_i32(bufs[i & 1], out=bufs[(i + 1) & 1])
+        return R.fold_sum_i32(bufs[iters & 1])
+
+    t, reps = _cpu_time(run, reps=3, budget_s=10.0)
+    live = run()
+    res = {"ms": round(t * 1e3, 2), "us_per_step": round(t * 1e6 / iters, 1), "cells_per_s": round(n * iters / t, 1),
+           "cores": threads, "kind": "port", "runs": reps, "live_cells_match_gpu": int(live) == gpu.get("live_cells"),
+           "gpu_over_cpu": round(t * 1e3 / gpu["ms"], 1),
+           "sample": "oracle Game of Life, the full 1024x1024 grid x 100 iterations + fold(sum)"}
+    log(f"[bench] cpu gol {res}")
+    return res
+
+
+def cpu_heat(R, threads, gpu, extra):
+    """The full 512^3 x 200 heat run with a residual every 10 on the oracle
+    (timed), checked bit for bit against the GPU's final state and residuals;
+    then the FP64 recurrence on the same inputs gives the error the FP32 run
+    carries after 200 steps."""
+    nx = ny = nz = 512
+    iters, every = 200, 10
+    n = nx * ny * nz
+    x0 = R.fill_uniform_f32(n, SEED, R.SCALE_UNIT).reshape(nz, ny, nx)
+    t0 = time.perf_counter()
+    ref, ref_res = R.heat7_iterate_f32(x0, iters, every)
+    t = time.perf_counter() - t0
+    gbs = 8 * n * iters / t / 1e9
+    res = {"ms": round(t * 1e3, 1), "GBs": round(gbs, 2), "cores": threads, "kind": "port",
+           "gpu_over_cpu": round(gpu["GBs"] / gbs, 1),
+           "sample": "oracle heat7, the full 512^3 cube x 200 iterations with the fused residual every 10"}
+    got = extra.get("heat_final")
+    if got is not None:
+        res["bit_exact_vs_gpu"] = bool(np.array_equal(got.reshape(ref.shape), ref)) and \
+            list(ref_res) == list(gpu["residuals"])
+        del ref
+        t1 = time.perf_counter()
+        x64 = R.heat7_iterate_f64(x0, iters)
+        d = np.abs(got.reshape(x64.shape).astype(np.float64) - x64)
+        res["fp64_error"] = {"max_abs": float(d.max()),
+                             "max_rel": float((d / np.maximum(np.abs(x64), 1e-30)).max()),
+                             "mean_abs": float(d.mean()), "fp64_ms": round((time.perf_counter() - t1) * 1e3, 1),
+                             "what": "GPU FP32 state after 200 steps vs an FP64 recurrence from the same inputs"}
+        del x64, d
+    log(f"[bench] cpu heat {res}")
+    return res
+
+
+def cpu_batch(R, threads, gpu, extra):
+    """All 4096 grids of the batch, 10 oracle mean-blur steps each plus the
+    FP64 oracle fold of each final state; the per-grid sums are compared with
+    the GPU's."""
+    from muesli4_amd.workloads import arena_offsets, batch_shapes
+    rows, cols, radii = batch_shapes(BATCH_GRIDS, BATCH_SEED)
+    off, total = arena_offsets(rows, cols)
+    arena = R.fill_uniform_f32(total, BATCH_SEED)
+    sums = np.empty(BATCH_GRIDS)
+    t0 = time.perf_counter()
+    for g in range(BATCH_GRIDS):
+        o = int(off[g])
+        x = arena[o:o + int(rows[g] * cols[g])].reshape(int(rows[g]), int(cols[g]))
+        sums[g] = R.fold_sum_f64(R.mean_blur_iterate_f32(x, int(radii[g]), BATCH_ITERS))
+    t = time.perf_counter() - t0
+    gbs = gpu["algorithmic_bytes"] / t / 1e9
+    res = {"ms": round(t * 1e3, 1), "GBs": round(gbs, 2), "cores": threads, "kind": "port",
+           "gpu_over_cpu": round(t * 1e3 / gpu["persistent_ms"], 1),
+           "sample": "oracle, every grid of the batch x 10 steps + FP64 fold, grids one after another"}
+    gs = extra.get("batch_sums")
+    if gs is not None:
+        res["max_rel_sum_diff_vs_gpu"] = float(np.max(np.abs(gs - sums) / np.maximum(np.abs(sums), 1e-30)))
+    log(f"[bench] cpu batch {res}")
+    return res
+
+
+def cpu_serve(R, threads, gpu, extra):
+    """The saturated serving mix on the oracle, jobs in trace order until
+    about 10 s have elapsed: CPU jobs/s on that sample beside the engine."""
+    from muesli4_amd.workloads import JOB_BLUR2D, JOB_HEAT3D, serve_trace
+    jobs, _ = serve_trace(SERVE_JOBS, SERVE_SEED, 0.0)
+    done = 0
+    t0 = time.perf_counter()
+    for d in jobs:
+        if d["kind"] == JOB_BLUR2D:
+            x = R.fill_uniform_f32(d["d0"] * d["d1"], d["seed"]).reshape(d["d0"], d["d1"])
+            R.fold_sum_f64(R.mean_blur_iterate_f32(x, d["r"], d["steps"], d.get("nv", 0.0)))
+        elif d["kind"] == JOB_HEAT3D:
+            x = R.fill_uniform_f32(d["d0"] * d["d1"] * d["d2"], d["seed"], R.SCALE_UNIT)
+            R.fold_sum_f64(R.heat7_iterate_f32(x.reshape(d["d2"], d["d1"], d["d0"]), d["steps"])[0])
+        else:
+            R.fold_sum_u32(R.map_affine_u32(R.fill_u32(d["n"], d["seed"]), d["mul"], d["add"]))
+        done += 1
+        if time.perf_counter() - t0 > 10.0 and done >= 5:
+            break
+    t = time.perf_counter() - t0
+    rate = done / t
+    eng = gpu["saturated"]["engine"]["jobs_per_s"]
+    res = {"jobs_per_s": round(rate, 2), "jobs": done, "s": round(t, 2), "cores": threads, "kind": "port",
+           "gpu_engine_over_cpu": round(eng / rate, 1),
+           "sample": f"oracle, the first {done} jobs of the saturated {SERVE_JOBS}-job trace in order, "
+                     "one after another (inputs filled, steps, fold)"}
+    log(f"[bench] cpu serve {res}")
+    return res
 
 
 if __name__ == "__main__":

@@ -184,6 +184,10 @@ int msl_graph_destroy(void *graph);
  * instruction sequence; this runs both on all 2^32 float inputs on the
  * device and returns the number of bitwise mismatches (0 expected).   */
 int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches);
+/* The same check for the heat kernels' division by 6 (the multiply/FMA
+ * sequence plus the ties-to-even rule for quotients exactly halfway
+ * between two subnormals).                                              */
+int msl_selftest_div_six(uint64_t *h_mismatches);
 
 /* ------------------------------------------------------------------ */
 /* Batched mean blur.  No single reference counterpart: it replaces a loop

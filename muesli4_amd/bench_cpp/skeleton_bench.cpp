@@ -10,7 +10,11 @@
  *   map / zip / fold functors as in examples/da_test.cpp.
  * Prints one JSON object; rates count 8 bytes per cell per stencil step
  * (one read, one write), 8 B/element for map, 12 for zip, 4 for fold.
+ * Times are wall clock over back-to-back calls, except *_device: HIP events
+ * on the library stream around one call (the fold's wall time also holds
+ * the host round trip of its result).
  */
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <string>
@@ -69,6 +73,24 @@ public:
 
 static double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device time of one call (HIP events on the library stream around it),
+// median of reps: for calls that end with a host copy (fold), whose wall
+// time also holds the host round trip.
+template <typename F>
+static double device_timed(F f, int reps) {
+    f();
+    std::vector<float> ms;
+    for (int i = 0; i < reps; i++) {
+        float t = 0.0f;
+        msl_timer_start();
+        f();
+        msl_timer_stop(&t);
+        ms.push_back(t);
+    }
+    std::sort(ms.begin(), ms.end());
+    return ms[ms.size() / 2] * 1e-3;
 }
 
 template <typename F>
@@ -141,6 +163,7 @@ int main(int argc, char **argv) {
         rate("dm_map_functor_8192", 8 * cells, timed([&] { c.map(aff, a); }, 10));
         rate("dm_zip_functor_8192", 12 * cells, timed([&] { c.zip(a, b, add); }, 10));
         rate("dm_fold_functor_8192", 4 * cells, timed([&] { (void)a.fold(add); }, 10));
+        rate("dm_fold_functor_8192_device", 4 * cells, device_timed([&] { (void)a.fold(add); }, 10));
     }
     printf("{");
     for (size_t i = 0; i < out.size(); i++) printf("%s%s", i ? ", " : "", out[i].c_str());

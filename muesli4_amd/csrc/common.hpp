@@ -127,6 +127,22 @@ __device__ __forceinline__ float div_odd_square(float x, float d, float rcp) {
     return (r != 0.0f && r == r) ? q1 : q0;
 }
 
+// Correctly rounded x / 6 (the heat functor's division) by the sequence of
+// div_odd_square with rcp = RN(1/6), plus one case that sequence misses: a
+// quotient exactly halfway between two subnormals (x = (6k+3) * 2^-149, where
+// the remainder r = x - 6*q0 is +-3 * 2^-149 and the inexact reciprocal
+// always rounds away from q0), which must round to the even one of q0 and
+// q1 = q0 +- 2^-149.  Branch-free; checked bit-for-bit against IEEE
+// division on all 2^32 inputs by msl_selftest_div_six.
+__device__ __forceinline__ float div6(float x) {
+    constexpr float kRcp6 = 1.0f / 6.0f;
+    const float q0 = __fmul_rn(x, kRcp6);
+    const float r = __fmaf_rn(-q0, 6.0f, x);
+    const float q1 = __fmaf_rn(r, kRcp6, q0);
+    const bool keep_even_q0 = fabsf(r) == 0x1.8p-148f && (__float_as_uint(q0) & 1u) == 0u;
+    return (r != 0.0f && r == r && !keep_even_q0) ? q1 : q0;
+}
+
 // Remap blockIdx so that each XCD (blocks are dealt round-robin to the 8
 // XCDs) receives a contiguous range of logical tiles; neighbouring tiles
 // then share one L2.  Identity when the grid does not split evenly.

@@ -114,7 +114,7 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
                 v = __fadd_rn(v, col[j + 2][sc]);             // y+1
                 v = __fadd_rn(v, col[j + 1][sb]);             // z-1
                 v = __fadd_rn(v, col[j + 1][sa]);             // z+1
-                v = __fdiv_rn(v, 6.0f);
+                v = div6(v);                                  // v / 6, correctly rounded
                 // row j+1 of col is output row y0+j: in the plane exactly
                 // when its cell is, and then coff is its unclamped offset
                 if (z < z1 && ((cok >> (j + 1)) & 1u)) {
@@ -192,7 +192,7 @@ __device__ __forceinline__ void heat7x2_march(float *__restrict__ out, const flo
         v = __fadd_rn(v, b[sc]);             // y+1
         v = __fadd_rn(v, zl[sb]);            // z-1
         v = __fadd_rn(v, zh[sa]);            // z+1
-        return __fdiv_rn(v, 6.0f);
+        return div6(v);
     };
     // generation 1 of plane zi from input slots (zi-1, zi, zi+1) into g1 slot d
     auto gen1 = [&](int zi, int sb, int sc, int sa, int d) {

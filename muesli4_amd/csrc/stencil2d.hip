@@ -437,8 +437,10 @@ static std::string plan_signature(const std::vector<int> &plan) {
 // ---- exhaustive check of div_odd_square against IEEE division -------------
 __global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
-// Every 32-bit pattern x: compare div_odd_square(x) with x / d bit for bit
-// (NaN against NaN counts as equal); one atomic per block.
+// Every 32-bit pattern x: compare div_odd_square(x) (kSix: div6, the heat
+// division) with x / d bit for bit (NaN against NaN counts as equal);
+// one atomic per block.
+template <bool kSix>
 __global__ __launch_bounds__(256) void div_check_kernel(float d, float rcp, unsigned long long *count) {
     __shared__ unsigned int part[256];
     unsigned int bad = 0;
@@ -446,7 +448,12 @@ __global__ __launch_bounds__(256) void div_check_kernel(float d, float rcp, unsi
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32); i += stride) {
         const float x = __uint_as_float((unsigned int)i);
         const float ref = __fdiv_rn(x, d);
-        const float got = div_odd_square(x, d, rcp);
+        float got;
+        if constexpr (kSix) {
+            got = div6(x);
+        } else {
+            got = div_odd_square(x, d, rcp);
+        }
         const bool same = (__float_as_uint(ref) == __float_as_uint(got)) || (ref != ref && got != got);
         bad += same ? 0u : 1u;
     }
@@ -569,7 +576,7 @@ int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches) {
     MSL_HIP_TRY(hipMalloc(&d_count, sizeof(unsigned long long)));
     zero_u64_kernel<<<1, 1, 0, ctx().stream>>>(d_count);
     const float d = (float)((2 * r + 1) * (2 * r + 1));
-    div_check_kernel<<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(d, 1.0f / d, d_count);
+    div_check_kernel<false><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(d, 1.0f / d, d_count);
     rc = check_launch("msl_selftest_div_odd_square");
     unsigned long long h = 0;
     hipError_t e = hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, ctx().stream);
@@ -577,6 +584,25 @@ int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches) {
     (void)hipFree(d_count);
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "msl_selftest_div_odd_square");
+    *h_mismatches = h;
+    return MSL_OK;
+}
+
+int msl_selftest_div_six(uint64_t *h_mismatches) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    MSL_REQUIRE(h_mismatches != nullptr, "msl_selftest_div_six: null out-pointer");
+    unsigned long long *d_count = nullptr;
+    MSL_HIP_TRY(hipMalloc(&d_count, sizeof(unsigned long long)));
+    zero_u64_kernel<<<1, 1, 0, ctx().stream>>>(d_count);
+    div_check_kernel<true><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(6.0f, 1.0f / 6.0f, d_count);
+    rc = check_launch("msl_selftest_div_six");
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, ctx().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
+    (void)hipFree(d_count);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "msl_selftest_div_six");
     *h_mismatches = h;
     return MSL_OK;
 }

@@ -109,6 +109,7 @@ SIGNATURES = {
     "msl_heat7_iterate_f32": [p, p, i32, i32, i32, f32, i32, i32, PF, PI],
     "msl_heat7_iterate_plan": [i32, i32, i32, i32, i32, PI, i32, PI],
     "msl_selftest_div_odd_square": [i32, C.POINTER(C.c_uint64)],
+    "msl_selftest_div_six": [C.POINTER(C.c_uint64)],
     "msl_graph_begin": [],
     "msl_graph_end": [PP],
     "msl_graph_launch": [p],

@@ -62,20 +62,61 @@ inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
     for (size_t i = (start) + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
          i += (size_t)gridDim.x * blockDim.x)
 
+// Grid of the vectorised streams (map, zip, fold): every lane keeps
+// kStreamUnroll 16-byte accesses in flight, so `per_cu` 256-thread blocks
+// per CU are enough to saturate HBM (one for map/zip, two for the read-only
+// fold stream, as the library's fixed-operation kernels).
+constexpr int kStreamUnroll = 4;
+inline int stream_grid(size_t n, int per_cu) {
+    const size_t per_block = (size_t)kThreads * 4 * kStreamUnroll;
+    size_t want = (n + per_block - 1) / per_block;
+    size_t cap = (size_t)num_cus() * per_cu;
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (int)want;
+}
+
+// Non-temporal 16-byte store of a quad of a 4-byte type: streamed outputs do
+// not evict the input being read from the Infinity Cache.
+template <typename R>
+__device__ __forceinline__ void store_quad_nt(Quad4<R> *p, const Quad4<R> &o) {
+    typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
+    v4u32 v;
+    __builtin_memcpy(&v, &o, sizeof v);
+    __builtin_nontemporal_store(v, reinterpret_cast<v4u32 *>(p));
+}
+
 // out[i] = f(in[i]); with quad (4-byte types, 16-byte aligned buffers) four
-// elements per 16-byte load and store, the remainder element-wise.
+// elements per 16-byte access and kStreamUnroll quads in flight per lane,
+// the remainder element-wise.
 template <typename T, typename R, typename F>
 __global__ __launch_bounds__(kThreads) void map_kernel(R *out, const T *in, size_t n, F f, int quad) {
     size_t start = 0;
     if constexpr (quad_type<T>() && quad_type<R>()) {
         if (quad) {
             const size_t n4 = n / 4;
-            MSL_GRID_STRIDE(i, n4) {
-                const Quad4<T> a = reinterpret_cast<const Quad4<T> *>(in)[i];
+            const Quad4<T> *in4 = reinterpret_cast<const Quad4<T> *>(in);
+            Quad4<R> *out4 = reinterpret_cast<Quad4<R> *>(out);
+            const size_t stride = (size_t)gridDim.x * blockDim.x;
+            size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+            for (; i + (kStreamUnroll - 1) * stride < n4; i += kStreamUnroll * stride) {
+                Quad4<T> a[kStreamUnroll];
+#pragma unroll
+                for (int u = 0; u < kStreamUnroll; u++) a[u] = in4[i + u * stride];
+#pragma unroll
+                for (int u = 0; u < kStreamUnroll; u++) {
+                    Quad4<R> o;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) o.v[k] = f.F::operator()(a[u].v[k]);
+                    store_quad_nt(out4 + i + u * stride, o);
+                }
+            }
+            for (; i < n4; i += stride) {
+                const Quad4<T> a = in4[i];
                 Quad4<R> o;
 #pragma unroll
                 for (int k = 0; k < 4; k++) o.v[k] = f.F::operator()(a.v[k]);
-                reinterpret_cast<Quad4<R> *>(out)[i] = o;
+                store_quad_nt(out4 + i, o);
             }
             start = n4 * 4;
         }
@@ -90,13 +131,34 @@ __global__ __launch_bounds__(kThreads) void zip_kernel(R *out, const A *a, const
     if constexpr (quad_type<A>() && quad_type<B>() && quad_type<R>()) {
         if (quad) {
             const size_t n4 = n / 4;
-            MSL_GRID_STRIDE(i, n4) {
-                const Quad4<A> x = reinterpret_cast<const Quad4<A> *>(a)[i];
-                const Quad4<B> y = reinterpret_cast<const Quad4<B> *>(b)[i];
+            const Quad4<A> *a4 = reinterpret_cast<const Quad4<A> *>(a);
+            const Quad4<B> *b4 = reinterpret_cast<const Quad4<B> *>(b);
+            Quad4<R> *out4 = reinterpret_cast<Quad4<R> *>(out);
+            const size_t stride = (size_t)gridDim.x * blockDim.x;
+            size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+            for (; i + (kStreamUnroll - 1) * stride < n4; i += kStreamUnroll * stride) {
+                Quad4<A> x[kStreamUnroll];
+                Quad4<B> y[kStreamUnroll];
+#pragma unroll
+                for (int u = 0; u < kStreamUnroll; u++) {
+                    x[u] = a4[i + u * stride];
+                    y[u] = b4[i + u * stride];
+                }
+#pragma unroll
+                for (int u = 0; u < kStreamUnroll; u++) {
+                    Quad4<R> o;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) o.v[k] = f.F::operator()(x[u].v[k], y[u].v[k]);
+                    store_quad_nt(out4 + i + u * stride, o);
+                }
+            }
+            for (; i < n4; i += stride) {
+                const Quad4<A> x = a4[i];
+                const Quad4<B> y = b4[i];
                 Quad4<R> o;
 #pragma unroll
                 for (int k = 0; k < 4; k++) o.v[k] = f.F::operator()(x.v[k], y.v[k]);
-                reinterpret_cast<Quad4<R> *>(out)[i] = o;
+                store_quad_nt(out4 + i, o);
             }
             start = n4 * 4;
         }
@@ -287,32 +349,85 @@ __global__ __launch_bounds__(kThreads) void zip_index_3d_kernel(R *out, const A 
 }
 
 // ---- fold ----------------------------------------------------------------------
-// Every thread folds its grid-stride elements in index order; threads and
-// blocks then combine pairwise in a fixed tree.  No identity element is
-// needed: an empty thread carries a "has no value" flag.
+// Every thread folds its grid-stride elements in index order; the lanes of a
+// wave then combine in a fixed pairwise tree by shuffles (lane l takes lane
+// l+m at m = 1, 2, 4, ..., so lane 0 ends with the in-order tree), and the
+// four wave results combine in wave order.  No identity element is needed:
+// an empty thread carries a "has no value" flag.  Types whose size is not a
+// multiple of 4 bytes combine through LDS in the same tree instead.
+template <typename T>
+constexpr bool shuffle_type() {
+    return sizeof(T) % 4 == 0 && sizeof(T) <= 64 && std::is_trivially_copyable<T>::value;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_down_words(const T &v, int m) {
+    constexpr int W = sizeof(T) / 4;
+    int w[W];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int j = 0; j < W; j++) w[j] = __shfl_down(w[j], m);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
 template <typename T, typename F>
 __device__ __forceinline__ void fold_block(T acc, bool has, F &f, T *out, int *out_has) {
-    __shared__ T sv[kThreads];
-    __shared__ int sh[kThreads];
-    sv[threadIdx.x] = acc;
-    sh[threadIdx.x] = has ? 1 : 0;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < (unsigned)s) {
-            const int o = threadIdx.x + s;
-            if (sh[o]) {
-                sv[threadIdx.x] = sh[threadIdx.x] ? f.F::operator()(sv[threadIdx.x], sv[o]) : sv[o];
-                sh[threadIdx.x] = 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ __attribute__((aligned(16))) unsigned char wv_raw[(kThreads / 64) * sizeof(T)];
+    __shared__ int wh[kThreads / 64];
+    T *wv = reinterpret_cast<T *>(wv_raw);
+    int h = has ? 1 : 0;
+    if constexpr (shuffle_type<T>()) {
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const T o = shfl_down_words(acc, m);
+            const int oh = __shfl_down(h, m);
+            if ((lane & (2 * m - 1)) == 0 && lane + m < 64 && oh) {
+                acc = h ? f.F::operator()(acc, o) : o;
+                h = 1;
             }
         }
+    } else {
+        __shared__ __attribute__((aligned(16))) unsigned char sv_raw[kThreads * sizeof(T)];
+        __shared__ int sh[kThreads];
+        T *sv = reinterpret_cast<T *>(sv_raw);
+        sv[threadIdx.x] = acc;
+        sh[threadIdx.x] = h;
         __syncthreads();
+        for (int m = 1; m < 64; m <<= 1) {
+            if ((lane & (2 * m - 1)) == 0 && sh[threadIdx.x + m]) {
+                sv[threadIdx.x] = sh[threadIdx.x] ? f.F::operator()(sv[threadIdx.x], sv[threadIdx.x + m])
+                                                  : sv[threadIdx.x + m];
+                sh[threadIdx.x] = 1;
+            }
+            __syncthreads();
+        }
+        acc = sv[threadIdx.x];
+        h = sh[threadIdx.x];
     }
+    if (lane == 0) {
+        wv[wave] = acc;
+        wh[wave] = h;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        *out = sv[0];
-        *out_has = sh[0];
+        T r = wv[0];
+        int rh = wh[0];
+        for (int w = 1; w < kThreads / 64; w++) {
+            if (wh[w]) {
+                r = rh ? f.F::operator()(r, wv[w]) : wv[w];
+                rh = 1;
+            }
+        }
+        *out = r;
+        *out_has = rh;
     }
 }
 
+// Stage 1: kStreamUnroll 16-byte loads in flight per lane (4-byte types on
+// aligned buffers), then the element tail; one partial per block.
 template <typename T, typename F>
 __global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size_t n, T *partials, int *has, F f,
                                                                int quad) {
@@ -322,8 +437,24 @@ __global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size
     if constexpr (quad_type<T>()) {
         if (quad) {  // this thread's quads in grid-stride order, then its tail elements
             const size_t n4 = n / 4;
-            MSL_GRID_STRIDE(i, n4) {
-                const Quad4<T> a = reinterpret_cast<const Quad4<T> *>(in)[i];
+            const Quad4<T> *in4 = reinterpret_cast<const Quad4<T> *>(in);
+            const size_t stride = (size_t)gridDim.x * blockDim.x;
+            size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+            for (; i + (kStreamUnroll - 1) * stride < n4; i += kStreamUnroll * stride) {
+                Quad4<T> a[kStreamUnroll];
+#pragma unroll
+                for (int u = 0; u < kStreamUnroll; u++) a[u] = in4[i + u * stride];
+#pragma unroll
+                for (int u = 0; u < kStreamUnroll; u++) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        acc = h ? f.F::operator()(acc, a[u].v[k]) : a[u].v[k];
+                        h = true;
+                    }
+                }
+            }
+            for (; i < n4; i += stride) {
+                const Quad4<T> a = in4[i];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     acc = h ? f.F::operator()(acc, a.v[k]) : a.v[k];
@@ -366,6 +497,16 @@ __global__ __launch_bounds__(kThreads) void fold_stage2_kernel(const T *partials
 // the current step reads, so one barrier per step suffices.
 constexpr int kMarchW = 256;
 constexpr int kMarchK = 4;
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every outstanding global load of the wave, which would drain the
+// rows (planes) that the marches request a step ahead at every step; the
+// "local"-only fences make the barrier wait for LDS operations alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 constexpr size_t kMaxLdsBytes = 48 * 1024;
 
 constexpr int pow2_at_least(int v) {
@@ -438,7 +579,7 @@ __device__ __forceinline__ void march2d(const T *__restrict__ in, int rows, int 
             qa[k] = load(yy + kMarchK, xa);
             qb[k] = load(yy + kMarchK, xb);
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int k = 0; k < kMarchK; k++) {
             const int yo = a - r + k;  // its window rows a-2r+k .. a+k are staged
@@ -717,7 +858,7 @@ __global__ __launch_bounds__(kThreads) void dc_stencil_kernel(T *out, const T *i
                 q[d][k] = load(k, zz + kMarch3K);
             }
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int d = 0; d < kMarch3K; d++) {
             const int zo = a - r + d;  // its window planes a-2r+d .. a+d are staged

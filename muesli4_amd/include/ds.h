@@ -145,7 +145,7 @@ public:
     T fold(FoldFunctor &f, bool /*final_fold_on_cpu*/ = true) {
         if (n == 0) throw detail::NotSameSizeException();
         updateDevice();
-        const int blocks = detail::grid_for(n, 4);
+        const int blocks = detail::stream_grid(n, 2);
         ensureFoldScratch(blocks);
         detail::fold_stage1_kernel<T, FoldFunctor>
             <<<blocks, detail::kThreads, 0, detail::stream()>>>(d_data, (size_t)n, d_part, d_has, f,
@@ -373,15 +373,19 @@ protected:
     template <typename F>
     void launch_map(T *out, const T *in, F &f) {
         if (n == 0) return;
-        detail::map_kernel<T, T, F><<<detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(
-            out, in, (size_t)n, f, detail::aligned16(out) && detail::aligned16(in));
+        const bool quad = detail::aligned16(out) && detail::aligned16(in);
+        detail::map_kernel<T, T, F>
+            <<<quad ? detail::stream_grid(n, 1) : detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(
+                out, in, (size_t)n, f, quad);
         detail::check_launch("DS::map");
     }
     template <typename A, typename B, typename F>
     void launch_zip(T *out, const A *a, const B *b, F &f) {
         if (n == 0) return;
-        detail::zip_kernel<A, B, T, F><<<detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(
-            out, a, b, (size_t)n, f, detail::aligned16(out) && detail::aligned16(a) && detail::aligned16(b));
+        const bool quad = detail::aligned16(out) && detail::aligned16(a) && detail::aligned16(b);
+        detail::zip_kernel<A, B, T, F>
+            <<<quad ? detail::stream_grid(n, 1) : detail::grid_for(n), detail::kThreads, 0, detail::stream()>>>(
+                out, a, b, (size_t)n, f, quad);
         detail::check_launch("DS::zip");
     }
 

@@ -222,6 +222,10 @@ def test_fast_division_matches_ieee_on_all_inputs():
         bad = C.c_uint64(1)
         call("msl_selftest_div_odd_square", r, C.byref(bad))
         assert bad.value == 0, (r, bad.value)
+    note("div selftest d=6 (heat)")
+    bad = C.c_uint64(1)
+    call("msl_selftest_div_six", C.byref(bad))
+    assert bad.value == 0, bad.value
 
 
 @pytest.mark.parametrize("r", [11, 13, 16])
