@@ -181,8 +181,10 @@ int msl_graph_destroy(void *graph);
 /* ------------------------------------------------------------------ */
 /* Diagnostics (no reference counterpart).  The mean-blur kernels divide
  * by (2r+1)^2 with a multiply/FMA sequence instead of the IEEE division
- * instruction sequence; this runs both on all 2^32 float inputs on the
- * device and returns the number of bitwise mismatches (0 expected).   */
+ * instruction sequence; this runs both forms (x / d, and (0 + x) / d for
+ * column sums formed without their leading +0) against IEEE division on
+ * all 2^32 float inputs on the device and returns the number of bitwise
+ * mismatches (0 expected).                                              */
 int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches);
 /* The same check for the heat kernels' division by 6 (the multiply/FMA
  * sequence plus the ties-to-even rule for quotients exactly halfway

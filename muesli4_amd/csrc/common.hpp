@@ -127,6 +127,21 @@ __device__ __forceinline__ float div_odd_square(float x, float d, float rcp) {
     return (r != 0.0f && r == r) ? q1 : q0;
 }
 
+// Correctly rounded (0 + x) / d for the same divisors: the finish of a sum
+// that the caller formed without its leading "0 +" (which only turns a -0
+// sum into +0).  The sequence of div_odd_square, with the remainder clamped
+// to a finite value instead of selecting q0: for infinite x the correction
+// then adds a finite term to q0 = +-inf, for NaN x it stays NaN, for x = -0
+// the remainder +0 gives +0 + -0 = +0, and for an exact quotient q0 + 0 =
+// q0.  Checked bit-for-bit against IEEE division of 0 + x on all 2^32
+// inputs by msl_selftest_div_odd_square.
+__device__ __forceinline__ float div_plus0(float x, float d, float rcp) {
+    const float q0 = __fmul_rn(x, rcp);
+    const float r = __fmaf_rn(-q0, d, x);
+    const float rc = __builtin_amdgcn_fmed3f(r, -3.0e38f, 3.0e38f);  // NaN -> -3e38
+    return __fmaf_rn(rc, rcp, q0);
+}
+
 // Correctly rounded x / 6 (the heat functor's division) by the sequence of
 // div_odd_square with rcp = RN(1/6), plus one case that sequence misses: a
 // quotient exactly halfway between two subnormals (x = (6k+3) * 2^-149, where
@@ -141,6 +156,18 @@ __device__ __forceinline__ float div6(float x) {
     const float q1 = __fmaf_rn(r, kRcp6, q0);
     const bool keep_even_q0 = fabsf(r) == 0x1.8p-148f && (__float_as_uint(q0) & 1u) == 0u;
     return (r != 0.0f && r == r && !keep_even_q0) ? q1 : q0;
+}
+
+// Correctly rounded (0 + x) / 6 for a sum formed without its leading "0 +":
+// div6 with the remainder clamped to a finite value as in div_plus0.
+// Checked on all 2^32 inputs by msl_selftest_div_six.
+__device__ __forceinline__ float div6_plus0(float x) {
+    constexpr float kRcp6 = 1.0f / 6.0f;
+    const float q0 = __fmul_rn(x, kRcp6);
+    const float r = __fmaf_rn(-q0, 6.0f, x);
+    const bool keep_even_q0 = fabsf(r) == 0x1.8p-148f && (__float_as_uint(q0) & 1u) == 0u;
+    const float q = __fmaf_rn(__builtin_amdgcn_fmed3f(r, -3.0e38f, 3.0e38f), kRcp6, q0);
+    return keep_even_q0 ? q0 : q;
 }
 
 // Remap blockIdx so that each XCD (blocks are dealt round-robin to the 8

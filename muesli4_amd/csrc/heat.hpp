@@ -107,14 +107,14 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
 #pragma unroll
             for (int j = 0; j < kTY; j++) {
                 const float c = col[j + 1][sc];
-                float v = 0.0f;
-                v = __fadd_rn(v, from_left(c, hal[j][sc]));   // x-1
+                // the sum without its leading "0 +", which div6_plus0 restores
+                float v = from_left(c, hal[j][sc]);           // x-1
                 v = __fadd_rn(v, from_right(c, hal[j][sc]));  // x+1
                 v = __fadd_rn(v, col[j][sc]);                 // y-1
                 v = __fadd_rn(v, col[j + 2][sc]);             // y+1
                 v = __fadd_rn(v, col[j + 1][sb]);             // z-1
                 v = __fadd_rn(v, col[j + 1][sa]);             // z+1
-                v = div6(v);                                  // v / 6, correctly rounded
+                v = div6_plus0(v);                            // (0 + v) / 6, correctly rounded
                 // row j+1 of col is output row y0+j: in the plane exactly
                 // when its cell is, and then coff is its unclamped offset
                 if (z < z1 && ((cok >> (j + 1)) & 1u)) {
@@ -185,14 +185,13 @@ __device__ __forceinline__ void heat7x2_march(float *__restrict__ out, const flo
         (void)sb;
         (void)sa;
         const float c = m[sc];
-        float v = 0.0f;
-        v = __fadd_rn(v, from_left(c, c));   // x-1
+        float v = from_left(c, c);           // x-1 (the leading "0 +" is in div6_plus0)
         v = __fadd_rn(v, from_right(c, c));  // x+1
         v = __fadd_rn(v, a[sc]);             // y-1
         v = __fadd_rn(v, b[sc]);             // y+1
         v = __fadd_rn(v, zl[sb]);            // z-1
         v = __fadd_rn(v, zh[sa]);            // z+1
-        return div6(v);
+        return div6_plus0(v);
     };
     // generation 1 of plane zi from input slots (zi-1, zi, zi+1) into g1 slot d
     auto gen1 = [&](int zi, int sb, int sc, int sa, int d) {

@@ -306,8 +306,9 @@ constexpr long kGolSmallCells = 1L << 20;
 // Game of Life kernel above: lane = column, wave = RPW rows, row sums
 // ((0 + left) + centre) + right with the side cells by DPP, column sums
 // ((0 + above) + row) + below with the wave edge rows through LDS, then the
-// exact division by 9 -- the operation order of the one-step kernel, so
-// every generation is bit-identical to it.  Cells outside the grid hold nv
+// exact division by 9 -- the operation order of the one-step kernel (the
+// leading "0 +" of both sums folded into div_plus0), so every generation is
+// bit-identical to it.  Cells outside the grid hold nv
 // at every generation.
 template <int K, int RPW>
 __global__ __launch_bounds__(256) void mean3_multi_kernel(float *__restrict__ out, const float *__restrict__ in,
@@ -332,8 +333,9 @@ __global__ __launch_bounds__(256) void mean3_multi_kernel(float *__restrict__ ou
         for (int i = 0; i < RPW; i++) {
             const float l = __int_as_float(lane_from_left(__float_as_int(c[i])));
             const float r = __int_as_float(lane_from_right(__float_as_int(c[i])));
-            // (no leading +0: the column sum below starts at +0, which
-            // makes a -0 row sum and a +0 one give the same result)
+            // (no leading +0: the column sum below is finished as if it
+            // started at +0, which makes a -0 row sum and a +0 one give the
+            // same result)
             rs[i + 1] = __fadd_rn(__fadd_rn(l, c[i]), r);
         }
         edge[wave][0][lane] = rs[1];
@@ -344,9 +346,10 @@ __global__ __launch_bounds__(256) void mean3_multi_kernel(float *__restrict__ ou
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < RPW; i++) {
-            const float t = __fadd_rn(__fadd_rn(__fadd_rn(0.0f, rs[i]), rs[i + 1]), rs[i + 2]);
+            // the column sum without its leading +0, which div_plus0 restores
+            const float t = __fadd_rn(__fadd_rn(rs[i], rs[i + 1]), rs[i + 2]);
             const int gy = gy0 + i;
-            c[i] = (colok && (unsigned)gy < (unsigned)rows) ? div_odd_square(t, denom, rcp) : nv;
+            c[i] = (colok && (unsigned)gy < (unsigned)rows) ? div_plus0(t, denom, rcp) : nv;
         }
     }
     if (lane >= K && lane < 64 - K && colok) {
@@ -437,20 +440,26 @@ static std::string plan_signature(const std::vector<int> &plan) {
 // ---- exhaustive check of div_odd_square against IEEE division -------------
 __global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
-// Every 32-bit pattern x: compare div_odd_square(x) (kSix: div6, the heat
-// division) with x / d bit for bit (NaN against NaN counts as equal);
+// Every 32-bit pattern x: compare div_odd_square(x) (or div6, the heat
+// division, or div_plus0 against (0 + x) / d) with x / d bit for bit (NaN against NaN counts as equal);
 // one atomic per block.
-template <bool kSix>
+// kKind 0: div_odd_square, 1: div6, 2: div_plus0 and 3: div6_plus0, the last
+// two against (0 + x) / d
+template <int kKind>
 __global__ __launch_bounds__(256) void div_check_kernel(float d, float rcp, unsigned long long *count) {
     __shared__ unsigned int part[256];
     unsigned int bad = 0;
     const uint64_t stride = (uint64_t)gridDim.x * 256;
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32); i += stride) {
         const float x = __uint_as_float((unsigned int)i);
-        const float ref = __fdiv_rn(x, d);
+        const float ref = __fdiv_rn(kKind >= 2 ? __fadd_rn(0.0f, x) : x, d);
         float got;
-        if constexpr (kSix) {
+        if constexpr (kKind == 1) {
             got = div6(x);
+        } else if constexpr (kKind == 2) {
+            got = div_plus0(x, d, rcp);
+        } else if constexpr (kKind == 3) {
+            got = div6_plus0(x);
         } else {
             got = div_odd_square(x, d, rcp);
         }
@@ -576,7 +585,8 @@ int msl_selftest_div_odd_square(int r, uint64_t *h_mismatches) {
     MSL_HIP_TRY(hipMalloc(&d_count, sizeof(unsigned long long)));
     zero_u64_kernel<<<1, 1, 0, ctx().stream>>>(d_count);
     const float d = (float)((2 * r + 1) * (2 * r + 1));
-    div_check_kernel<false><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(d, 1.0f / d, d_count);
+    div_check_kernel<0><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(d, 1.0f / d, d_count);
+    div_check_kernel<2><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(d, 1.0f / d, d_count);
     rc = check_launch("msl_selftest_div_odd_square");
     unsigned long long h = 0;
     hipError_t e = hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, ctx().stream);
@@ -595,7 +605,8 @@ int msl_selftest_div_six(uint64_t *h_mismatches) {
     unsigned long long *d_count = nullptr;
     MSL_HIP_TRY(hipMalloc(&d_count, sizeof(unsigned long long)));
     zero_u64_kernel<<<1, 1, 0, ctx().stream>>>(d_count);
-    div_check_kernel<true><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(6.0f, 1.0f / 6.0f, d_count);
+    div_check_kernel<1><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(6.0f, 1.0f / 6.0f, d_count);
+    div_check_kernel<3><<<ctx().num_cus * 8, 256, 0, ctx().stream>>>(6.0f, 1.0f / 6.0f, d_count);
     rc = check_launch("msl_selftest_div_six");
     unsigned long long h = 0;
     hipError_t e = hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, ctx().stream);

@@ -203,11 +203,12 @@ struct MeanPolicy {
         for (int d = 1; d <= 2 * R; d++) s = __fadd_rn(s, w[RP - R + c + d]);
         return s;
     }
-    // Correctly rounded colsum / denom (identical to IEEE division).
+    // Correctly rounded (0 + colsum) / denom (identical to IEEE division of
+    // the definition's column sum, which starts at +0: the march sums the
+    // row sums from the first one, and the +0 only matters for a sum of -0).
     __device__ __forceinline__ float finish(float colsum, float /*center*/) const {
-        return div_odd_square(colsum, denom, rcp);
+        return div_plus0(colsum, denom, rcp);
     }
-    __device__ __forceinline__ float colsum_init() const { return 0.0f; }
     __device__ __forceinline__ float acc(float a, float b) const { return __fadd_rn(a, b); }
 };
 
@@ -228,7 +229,6 @@ struct LifePolicy {
         if (live == 0 && sum == 3) next = 1;
         return next;
     }
-    __device__ __forceinline__ int32_t colsum_init() const { return 0; }
     __device__ __forceinline__ int32_t acc(int32_t a, int32_t b) const { return a + b; }
 };
 
@@ -367,10 +367,11 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     // rows y-R .. y+R in ascending order: the oldest ring entry
-                    // first when marching down, the newest first when marching up
-                    T acc = pol.colsum_init();
+                    // first when marching down, the newest first when marching
+                    // up (from the first row sum: see MeanPolicy::finish)
+                    T acc = ring[UP ? u : (u + P - (PER - 1)) % P].v[c];
 #pragma unroll
-                    for (int k = 0; k < PER; k++)
+                    for (int k = 1; k < PER; k++)
                         acc = pol.acc(acc, ring[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
                     T centre = T(0);
                     if constexpr (Pol::kCenter) centre = cring[(u + P - R) % P].v[c];
@@ -493,9 +494,9 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
             VecN<float, C> v1;
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                float acc = pol.colsum_init();
+                float acc = ring1[UP ? u : (u + P - (PER - 1)) % P].v[c];
 #pragma unroll
-                for (int k = 0; k < PER; k++)
+                for (int k = 1; k < PER; k++)
                     acc = pol.acc(acc, ring1[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
                 v1.v[c] = (row1 && col_in[c]) ? pol.finish(acc, 0.0f) : nv;
             }
@@ -520,9 +521,9 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
             VecN<float, C> o;
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                float acc = pol.colsum_init();
+                float acc = ring2[UP ? u : (u + P - (PER - 1)) % P].v[c];
 #pragma unroll
-                for (int k = 0; k < PER; k++)
+                for (int k = 1; k < PER; k++)
                     acc = pol.acc(acc, ring2[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
                 o.v[c] = pol.finish(acc, 0.0f);
             }

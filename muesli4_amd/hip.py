@@ -8,7 +8,9 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "lib", "libmsl_hip.so")
+# MSL_HIP_LIB names another build of the library (e.g. to compare two
+# builds in one process tree); by default the in-tree build is loaded.
+_LIB_PATH = os.environ.get("MSL_HIP_LIB") or os.path.join(_HERE, "lib", "libmsl_hip.so")
 
 MSL_OK = 0
 ERROR_NAMES = {-1: "MSL_E_INVALID", -2: "MSL_E_HIP", -3: "MSL_E_NOMEM", -4: "MSL_E_STATE",
