@@ -27,8 +27,16 @@ constexpr int kStripW = StripConsts<1, 4>::SW;  // columns per wave strip
 // wave-level ordering is needed (no workgroup barrier).  C columns per lane:
 // 4 (256-column strips) in general, 1 (64-column strips) for small grids,
 // where four times as many strips keep more waves busy.
+// Radii 9 and 10 with 256-column strips: at most two waves per SIMD (the
+// register budget of the wave-uniform row addressing would allow three, but
+// a third wave per SIMD leaves a partial last round of the 16 waves per CU
+// that the segment shape aims at).
+template <int R, int C>
+constexpr int strip_max_waves_per_eu() { return (C == 4 && R >= 9 && R <= 10) ? 2 : 8; }
+
 template <typename Pol, int MODE, bool NT = true, int C = 4>
-__global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__restrict__ out,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, strip_max_waves_per_eu<Pol::R, C>())))
+void strip_stencil_kernel(typename Pol::T *__restrict__ out,
                                                             const typename Pol::T *__restrict__ in, int rows, int cols,
                                                             typename Pol::T nv, int seg, int strips_x, int nitems,
                                                             Pol pol) {
@@ -37,10 +45,10 @@ __global__ __launch_bounds__(256) void strip_stencil_kernel(typename Pol::T *__r
     __shared__ __attribute__((aligned(16))) T lds[4][2][StripConsts<R, C>::LDSW];
 
     const int wave = threadIdx.x >> 6;
-    // (the item is wave-uniform, but telling the compiler so with
-    // readfirstlane costs r = 10 5 %: the scalar path then lengthens the
-    // dependent address chain of each row)
-    const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
+    // the item is wave-uniform: with readfirstlane the row indices, clamps
+    // and row bases of the march are scalar (r = 10: 205 -> 190 VALU
+    // instructions per marched row, 170 -> 154 VGPRs)
+    const int item = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave);
     if (item >= nitems) return;
     const int sx = item % strips_x;
     const int sy = item / strips_x;
