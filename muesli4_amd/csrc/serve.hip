@@ -18,17 +18,24 @@
 //     persistent launch covers all active blur jobs' (strip, segment) items,
 //     one covers all heat jobs' (tile, z-segment) items, one covers the
 //     chunks of the map+fold jobs admitted this round; a job's state stays
-//     resident in the arena across rounds;
-//   * a job's last step folds its final state (compensated FP32 pairs per
-//     item, combined in FP64; wrapping u32 for map+fold) and the round's
-//     results are copied to pinned host memory;
+//     resident in the arena across rounds.  When the active jobs' items
+//     cannot fill the GPU (light load), a round advances each job by up to
+//     kMaxSub steps instead, one launch per step, so that the per-round
+//     host work and launches are spread over several steps;
+//   * after a job's last step its final state is folded (compensated FP32
+//     pairs per fixed 64K-element chunk, combined in FP64 in chunk order;
+//     wrapping u32 for map+fold) and the round's results are copied to
+//     pinned host memory;
 //   * up to kRing rounds are in flight; a round's event retires its
 //     finished jobs (results, latency stamps) and frees their memory, after
 //     which waiting jobs can be admitted.
 // Jobs that arrive while others are on the GPU join at the next round, so
 // the work of differently sized jobs overlaps inside the same launches.
-// Per-job tiling depends only on the job's shape, so every result is
-// bit-identical whether the job runs alone or in a mix.
+// Every stencil cell is computed in the operation order of the single-grid
+// kernels whatever the tiling, and the fold partition depends only on the
+// job's size, so every result is bit-identical whether the job runs alone
+// or in a mix.  The tiling itself follows the load: at light load the blur
+// segments are shortened until a job alone spreads over the device.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -48,6 +55,7 @@ int mean_step(float *out, const float *in, int rows, int cols, int r, float nv, 
 int heat_step(float *out, const float *in, int nx, int ny, int nz, float nv, float *maxdiff);
 
 constexpr int kRing = 3;             // rounds in flight
+constexpr int kMaxSub = 8;           // steps per round while the active items cannot fill the GPU
 constexpr int kMapChunk = 1 << 16;   // elements per map+fold work item
 constexpr int kHeatZseg = 16;        // z-planes per heat work item
 constexpr size_t kAlign = 256;
@@ -55,19 +63,27 @@ const float kScale255 = 255.0f / 16777216.0f;
 const float kScaleUnit = 1.0f / 16777216.0f;
 
 // ---- per-round descriptors (device) ------------------------------------------
+// A stencil job's step k of the round (k < nsub) reads state a when
+// parity0 + k is even (b otherwise) and writes the other one.
 struct SBlur {
-    const float *src;
-    float *dst;
+    float *a;
+    float *b;
     int rows, cols, r, kind;
     float nv;
-    int strips_x, seg, item0, nitems, fold;
+    int strips_x, seg, item0, nitems, parity0, nsub;
 };
 struct SHeat {
-    const float *src;
-    float *dst;
+    float *a;
+    float *b;
     int nx, ny, nz;
     float nv;
-    int xtiles, ytiles, zseg, item0, nitems, fold;
+    int xtiles, ytiles, zseg, item0, nitems, parity0, nsub;
+};
+// The final state of a finishing stencil job, folded in kMapChunk chunks.
+struct SSum {
+    const float *data;
+    long long n;
+    int item0, nitems;
 };
 struct SMap {
     uint32_t *data;
@@ -80,12 +96,12 @@ struct SFinal {
 };
 
 // item_desc[i] = largest j with starts[j] <= i (starts ascending, starts[0] = 0);
-// also clears the round's work-queue counter of this kind (the work kernel
-// runs after this one on the same stream).
+// also clears the round's nctr work-queue counters of this kind, one per step
+// of the round (the work kernels run after this one on the same stream).
 __global__ void serve_expand_kernel(const int *__restrict__ starts, int ndesc, int nitems, int *__restrict__ item_desc,
-                                    unsigned *__restrict__ ctr) {
+                                    unsigned *__restrict__ ctr, int nctr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *ctr = 0u;
+    if (i < nctr) ctr[i] = 0u;
     if (i >= nitems) return;
     int lo = 0, hi = ndesc - 1;
     while (lo < hi) {
@@ -105,16 +121,22 @@ __device__ __forceinline__ int serve_next_item(unsigned *ctr, int stride) {
     return stride + __builtin_amdgcn_readfirstlane((int)nx);
 }
 
+// Step k of the round over every item of every active blur job (items of
+// jobs with fewer steps in this round are skipped).
 __global__ __launch_bounds__(256) void serve_blur_kernel(const SBlur *__restrict__ descs,
                                                          const int *__restrict__ item_desc, int nitems,
-                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
+                                                         unsigned *__restrict__ ctr, int k_step) {
     __shared__ __attribute__((aligned(16))) float lds[4][2 * kBatchLdsw];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int nw = gridDim.x * 4;
     for (int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave); i < nitems;
          i = serve_next_item(ctr, nw)) {
         const int j = __builtin_amdgcn_readfirstlane(item_desc[i]);
         const SBlur g = descs[j];
+        if (k_step >= g.nsub) continue;
+        const bool odd = (g.parity0 + k_step) & 1;
+        const float *src = odd ? g.b : g.a;
+        float *dst = odd ? g.a : g.b;
         const int k = i - g.item0;
         const int sw = g.kind == kKindNarrow1 ? 64 : 256;
         const int xs = (k % g.strips_x) * sw;
@@ -122,25 +144,23 @@ __global__ __launch_bounds__(256) void serve_blur_kernel(const SBlur *__restrict
         const int y0 = sy * g.seg;
         const int y1 = min(y0 + g.seg, g.rows);
         LaneSum ls;
-        if (g.fold >= 0) {
-            blur_item_r<true>(g.r, g.src, g.dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls, sy & 1);
-            wave_pair_reduce(ls);
-            if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
-        } else {
-            blur_item_r<false>(g.r, g.src, g.dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls, sy & 1);
-        }
+        blur_item_r<false>(g.r, src, dst, g.rows, g.cols, g.nv, g.kind, xs, y0, y1, lds[wave], &ls, sy & 1);
     }
 }
 
 __global__ __launch_bounds__(256) void serve_heat_kernel(const SHeat *__restrict__ descs,
                                                          const int *__restrict__ item_desc, int nitems,
-                                                         float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                                                         unsigned *__restrict__ ctr, int k_step) {
+    const int wave = threadIdx.x >> 6;
     const int nw = gridDim.x * 4;
     for (int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave); i < nitems;
          i = serve_next_item(ctr, nw)) {
         const int j = __builtin_amdgcn_readfirstlane(item_desc[i]);
         const SHeat g = descs[j];
+        if (k_step >= g.nsub) continue;
+        const bool odd = (g.parity0 + k_step) & 1;
+        const float *src = odd ? g.b : g.a;
+        float *dst = odd ? g.a : g.b;
         const int k = i - g.item0;
         const int xt = k % g.xtiles;
         const int rest = k / g.xtiles;
@@ -148,15 +168,7 @@ __global__ __launch_bounds__(256) void serve_heat_kernel(const SHeat *__restrict
         const int z0 = (rest / g.ytiles) * g.zseg;
         const int z1 = min(z0 + g.zseg, g.nz);
         float dmax = 0.0f;
-        LaneSum ls;
-        if (g.fold >= 0) {
-            heat7_march<false, true>(g.dst, g.src, g.nx, g.ny, g.nz, g.nv, xt * kTX, yt * kTY, z0, z1, &dmax, &ls);
-            wave_pair_reduce(ls);
-            if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
-        } else {
-            heat7_march<false, false>(g.dst, g.src, g.nx, g.ny, g.nz, g.nv, xt * kTX, yt * kTY, z0, z1, &dmax,
-                                      nullptr);
-        }
+        heat7_march<false, false>(dst, src, g.nx, g.ny, g.nz, g.nv, xt * kTX, yt * kTY, z0, z1, &dmax, nullptr);
     }
 }
 
@@ -194,11 +206,45 @@ __global__ __launch_bounds__(256) void serve_map_kernel(const SMap *__restrict__
     }
 }
 
-// One wave per finishing job: FP64 sum of its (s, c) pairs in item order, or
-// the wrapping sum of its u32 partials.
+// Compensated sum of one kMapChunk chunk of a finished stencil job's final
+// state: each lane pushes its elements (16-byte loads, lane-strided) into a
+// TwoSum pair, the wave combines the lanes in a fixed butterfly.
+__global__ __launch_bounds__(256) void serve_sum_kernel(const SSum *__restrict__ descs,
+                                                        const int *__restrict__ item_desc, int nitems,
+                                                        float2 *__restrict__ partials) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = gridDim.x * 4;
+    for (int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave); i < nitems;
+         i += nw) {
+        const int j = __builtin_amdgcn_readfirstlane(item_desc[i]);
+        const SSum g = descs[j];
+        const long long e0 = (long long)(i - g.item0) * kMapChunk;
+        const long long e1 = min(e0 + (long long)kMapChunk, g.n);
+        LaneSum ls;
+        long long e = e0;
+        if ((((uintptr_t)(g.data + e0)) & 15u) == 0) {
+            typedef float v4f_t __attribute__((ext_vector_type(4)));
+            const long long nq = (e1 - e0) / 4;
+            const v4f_t *q = reinterpret_cast<const v4f_t *>(g.data + e0);
+            for (long long t = lane; t < nq; t += 64) {
+                const v4f_t v = q[t];
+                ls.push(v.x);
+                ls.push(v.y);
+                ls.push(v.z);
+                ls.push(v.w);
+            }
+            e = e0 + nq * 4;
+        }
+        for (e += lane; e < e1; e += 64) ls.push(g.data[e]);
+        wave_pair_reduce(ls);
+        if (lane == 0) partials[i] = make_float2(ls.s, ls.c);
+    }
+}
+
+// One wave per finishing job: FP64 sum of its (s, c) chunk pairs in chunk
+// order, or the wrapping sum of its u32 partials.
 __global__ __launch_bounds__(256) void serve_final_kernel(const SFinal *__restrict__ finals, int nf,
-                                                          const float2 *__restrict__ pb,
-                                                          const float2 *__restrict__ ph,
+                                                          const float2 *__restrict__ ps,
                                                           const uint32_t *__restrict__ pm, double *__restrict__ rf,
                                                           uint32_t *__restrict__ ru) {
     const int lane = threadIdx.x & 63;
@@ -212,10 +258,9 @@ __global__ __launch_bounds__(256) void serve_final_kernel(const SFinal *__restri
         for (int m = 32; m >= 1; m >>= 1) acc += (uint32_t)__shfl_xor((int)acc, m);
         if (lane == 0) ru[F.slot] = acc;
     } else {
-        const float2 *p = F.kind == MSL_JOB_BLUR2D ? pb : ph;
         double acc = 0.0;
         for (int k = lane; k < F.nitems; k += 64) {
-            const float2 v = p[F.item0 + k];
+            const float2 v = ps[F.item0 + k];
             acc += (double)v.x;
             acc += (double)v.y;
         }
@@ -293,6 +338,7 @@ struct SJob {
     float *a = nullptr, *b = nullptr;
     uint32_t *u = nullptr;
     int steps_done = 0;
+    int nsub = 0;  // steps in the round being planned
     // tiling
     int kind = 0, strips_x = 0, seg = 0, xtiles = 0, ytiles = 0, nitems = 0;
     double sum = 0.0;
@@ -305,11 +351,11 @@ struct RoundSlot {
     size_t stage_cap = 0;
     char *h_stage = nullptr;  // pinned
     char *d_stage = nullptr;
-    int item_cap[3] = {0, 0, 0};
-    int *d_item_desc[3] = {nullptr, nullptr, nullptr};
-    float2 *d_pb = nullptr, *d_ph = nullptr;
+    int item_cap[4] = {0, 0, 0, 0};  // blur, heat, map, sum items
+    int *d_item_desc[4] = {nullptr, nullptr, nullptr, nullptr};
+    float2 *d_ps = nullptr;  // sum chunk partials
     uint32_t *d_pm = nullptr;
-    unsigned *d_ctr = nullptr;  // work-queue counters: blur, heat, map
+    unsigned *d_ctr = nullptr;  // work-queue counters, kMaxSub each: blur, heat, then map
     double *d_rf = nullptr, *h_rf = nullptr;
     uint32_t *d_ru = nullptr, *h_ru = nullptr;
     std::vector<std::pair<SJob *, int>> finishing;
@@ -364,18 +410,37 @@ static int validate_job(const msl_job &j, size_t budget) {
     return MSL_OK;
 }
 
-// Fixed per-shape tiling (so a job's results do not depend on its company).
+// A job's strip layout and (for the fold) chunk count, fixed at admission.
+// Blur segments of a round: 64 rows (32 for one column per lane) under
+// load; at light load (fine) short enough that a job alone spreads over the
+// device, about 4 waves per CU (a large grid served alone otherwise ran on an
+// eighth of the GPU), but at least 4 / 8 / 16 rows for r <= 2, <= 6 and
+// above so that the 2r halo rows stay a modest share.  The results do not
+// depend on the segments.
+static void tile_blur(SJob &J, bool fine) {
+    const msl_job &s = J.spec;
+    const int rows = s.d0;
+    const int seg_max = J.kind == kKindNarrow1 ? 32 : 64;
+    J.seg = seg_max;
+    if (fine) {
+        const long target = (long)ctx().num_cus * 4;
+        const long segs = (target + J.strips_x - 1) / J.strips_x;
+        const int seg_min = s.r <= 2 ? 4 : (s.r <= 6 ? 8 : 16);
+        J.seg = std::min(seg_max, std::max(seg_min, (int)((rows + segs - 1) / segs)));
+    }
+    J.nitems = J.strips_x * ((rows + J.seg - 1) / J.seg);
+}
+
 static void plan_job(SJob &J) {
     const msl_job &s = J.spec;
     if (s.kind == MSL_JOB_BLUR2D) {
-        const int rows = s.d0, cols = s.d1;
+        const int cols = s.d1;
         const bool narrow = cols < 192 || s.r > kBatchMaxR4 || cols < 4;
         const uintptr_t al = (uintptr_t)J.a | (uintptr_t)J.b;
         J.kind = narrow ? kKindNarrow1 : (((cols & 3) == 0 && (al & 15u) == 0) ? kKindAligned4 : kKindUnaligned4);
         const int sw = narrow ? 64 : 256;
-        J.seg = narrow ? 32 : 64;
         J.strips_x = (cols + sw - 1) / sw;
-        J.nitems = J.strips_x * ((rows + J.seg - 1) / J.seg);
+        tile_blur(J, false);
     } else if (s.kind == MSL_JOB_HEAT3D) {
         J.xtiles = (s.d0 + kTX - 1) / kTX;
         J.ytiles = (s.d1 + kTY - 1) / kTY;
@@ -399,10 +464,9 @@ static void slot_free(RoundSlot &R) {
     if (R.ev) (void)hipEventDestroy(R.ev);
     if (R.h_stage) (void)hipHostFree(R.h_stage);
     if (R.d_stage) (void)hipFree(R.d_stage);
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 4; k++)
         if (R.d_item_desc[k]) (void)hipFree(R.d_item_desc[k]);
-    if (R.d_pb) (void)hipFree(R.d_pb);
-    if (R.d_ph) (void)hipFree(R.d_ph);
+    if (R.d_ps) (void)hipFree(R.d_ps);
     if (R.d_pm) (void)hipFree(R.d_pm);
     if (R.d_ctr) (void)hipFree(R.d_ctr);
     if (R.d_rf) (void)hipFree(R.d_rf);
@@ -413,8 +477,8 @@ static void slot_free(RoundSlot &R) {
 }
 
 // Grows a slot's buffers (only called while the slot is not in flight).
-static int slot_reserve(RoundSlot &R, size_t stage_bytes, const int items[3]) {
-    if (!R.d_ctr) MSL_HIP_TRY(hipMalloc(&R.d_ctr, sizeof(unsigned) * 3));
+static int slot_reserve(RoundSlot &R, size_t stage_bytes, const int items[4]) {
+    if (!R.d_ctr) MSL_HIP_TRY(hipMalloc(&R.d_ctr, sizeof(unsigned) * 4 * kMaxSub));
     if (stage_bytes > R.stage_cap) {
         if (R.h_stage) MSL_HIP_TRY(hipHostFree(R.h_stage));
         if (R.d_stage) MSL_HIP_TRY(hipFree(R.d_stage));
@@ -425,24 +489,20 @@ static int slot_reserve(RoundSlot &R, size_t stage_bytes, const int items[3]) {
         MSL_HIP_TRY(hipMalloc(&R.d_stage, cap));
         R.stage_cap = cap;
     }
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         if (items[k] <= R.item_cap[k]) continue;
         const int cap = std::max(items[k] + items[k] / 2, 4096);
         if (R.d_item_desc[k]) MSL_HIP_TRY(hipFree(R.d_item_desc[k]));
         R.d_item_desc[k] = nullptr;
         MSL_HIP_TRY(hipMalloc(&R.d_item_desc[k], sizeof(int) * cap));
-        if (k == 0) {
-            if (R.d_pb) MSL_HIP_TRY(hipFree(R.d_pb));
-            R.d_pb = nullptr;
-            MSL_HIP_TRY(hipMalloc(&R.d_pb, sizeof(float2) * cap));
-        } else if (k == 1) {
-            if (R.d_ph) MSL_HIP_TRY(hipFree(R.d_ph));
-            R.d_ph = nullptr;
-            MSL_HIP_TRY(hipMalloc(&R.d_ph, sizeof(float2) * cap));
-        } else {
+        if (k == 2) {
             if (R.d_pm) MSL_HIP_TRY(hipFree(R.d_pm));
             R.d_pm = nullptr;
             MSL_HIP_TRY(hipMalloc(&R.d_pm, sizeof(uint32_t) * cap));
+        } else if (k == 3) {
+            if (R.d_ps) MSL_HIP_TRY(hipFree(R.d_ps));
+            R.d_ps = nullptr;
+            MSL_HIP_TRY(hipMalloc(&R.d_ps, sizeof(float2) * cap));
         }
         R.item_cap[k] = cap;
     }
@@ -577,36 +637,56 @@ static int run_round(Server &S, int *did_work) {
     std::vector<SBlur> db;
     std::vector<SHeat> dh;
     std::vector<SMap> dm;
+    std::vector<SSum> ds;
     std::vector<SFinal> fin;
-    std::vector<int> sb, sh, sm;
+    std::vector<int> sb, sh, sm, ss;
     std::vector<std::pair<SJob *, int>> finishing;
-    int items[3] = {0, 0, 0};
+    int items[4] = {0, 0, 0, 0};
+    // Light load: the active stencil jobs' items at the default tiling
+    // cannot fill one persistent launch (3 workgroups of 4 waves per CU).
+    // Then the blur segments are shortened and each job advances up to
+    // kMaxSub steps this round (per-round host work and launch latency would
+    // otherwise dominate); under load one step with the default tiling.
+    long stencil_items = 0;
+    for (SJob *J : S.blur) {
+        tile_blur(*J, false);
+        stencil_items += J->nitems;
+    }
+    for (SJob *J : S.heat) stencil_items += J->nitems;
+    const bool light = stencil_items < (long)ctx().num_cus * 12;
+    const int ksteps = light ? kMaxSub : 1;
+    if (light)
+        for (SJob *J : S.blur) tile_blur(*J, true);
+    // the fold of a finishing stencil job: its final state in kMapChunk chunks
+    auto add_sum = [&](SJob *J, const float *data, long long n, int slot) {
+        const int nchunks = (int)((n + kMapChunk - 1) / kMapChunk);
+        ds.push_back(SSum{data, n, items[3], nchunks});
+        ss.push_back(items[3]);
+        fin.push_back(SFinal{J->spec.kind, items[3], nchunks, slot});
+        finishing.emplace_back(J, slot);
+        items[3] += nchunks;
+    };
+    int nsub_max[2] = {0, 0};
     for (SJob *J : S.blur) {
         const msl_job &s = J->spec;
-        const bool odd = J->steps_done & 1;
-        const bool last = J->steps_done == s.steps - 1;
-        const int slot = last ? (int)fin.size() : -1;
-        db.push_back(SBlur{odd ? J->b : J->a, odd ? J->a : J->b, s.d0, s.d1, s.r, J->kind, s.nv, J->strips_x, J->seg,
-                           items[0], J->nitems, slot});
+        J->nsub = std::min(ksteps, s.steps - J->steps_done);
+        nsub_max[0] = std::max(nsub_max[0], J->nsub);
+        db.push_back(SBlur{J->a, J->b, s.d0, s.d1, s.r, J->kind, s.nv, J->strips_x, J->seg, items[0], J->nitems,
+                           J->steps_done & 1, J->nsub});
         sb.push_back(items[0]);
-        if (last) {
-            fin.push_back(SFinal{MSL_JOB_BLUR2D, items[0], J->nitems, slot});
-            finishing.emplace_back(J, slot);
-        }
+        if (J->steps_done + J->nsub == s.steps)
+            add_sum(J, (s.steps & 1) ? J->b : J->a, (long long)s.d0 * s.d1, (int)fin.size());
         items[0] += J->nitems;
     }
     for (SJob *J : S.heat) {
         const msl_job &s = J->spec;
-        const bool odd = J->steps_done & 1;
-        const bool last = J->steps_done == s.steps - 1;
-        const int slot = last ? (int)fin.size() : -1;
-        dh.push_back(SHeat{odd ? J->b : J->a, odd ? J->a : J->b, s.d0, s.d1, s.d2, s.nv, J->xtiles, J->ytiles,
-                           J->seg, items[1], J->nitems, slot});
+        J->nsub = std::min(ksteps, s.steps - J->steps_done);
+        nsub_max[1] = std::max(nsub_max[1], J->nsub);
+        dh.push_back(SHeat{J->a, J->b, s.d0, s.d1, s.d2, s.nv, J->xtiles, J->ytiles, J->seg, items[1], J->nitems,
+                           J->steps_done & 1, J->nsub});
         sh.push_back(items[1]);
-        if (last) {
-            fin.push_back(SFinal{MSL_JOB_HEAT3D, items[1], J->nitems, slot});
-            finishing.emplace_back(J, slot);
-        }
+        if (J->steps_done + J->nsub == s.steps)
+            add_sum(J, (s.steps & 1) ? J->b : J->a, (long long)s.d0 * s.d1 * s.d2, (int)fin.size());
         items[1] += J->nitems;
     }
     for (SJob *J : maps) {
@@ -620,9 +700,9 @@ static int run_round(Server &S, int *did_work) {
     }
 
     // ---- stage and upload
-    const size_t stage_bytes = 16 * 8 + sizeof(SBlur) * db.size() + sizeof(SHeat) * dh.size() +
-                               sizeof(SMap) * dm.size() + sizeof(SFinal) * fin.size() +
-                               sizeof(int) * (sb.size() + sh.size() + sm.size());
+    const size_t stage_bytes = 16 * 10 + sizeof(SBlur) * db.size() + sizeof(SHeat) * dh.size() +
+                               sizeof(SMap) * dm.size() + sizeof(SSum) * ds.size() + sizeof(SFinal) * fin.size() +
+                               sizeof(int) * (sb.size() + sh.size() + sm.size() + ss.size());
     rc = slot_reserve(R, stage_bytes, items);
     if (rc) return rc;
     size_t off = 0;
@@ -630,41 +710,55 @@ static int run_round(Server &S, int *did_work) {
     SBlur *pb = stage_put(hb, off, db.data(), db.size());
     SHeat *ph = stage_put(hb, off, dh.data(), dh.size());
     SMap *pm = stage_put(hb, off, dm.data(), dm.size());
+    SSum *pss = stage_put(hb, off, ds.data(), ds.size());
     SFinal *pf = stage_put(hb, off, fin.data(), fin.size());
     int *psb = stage_put(hb, off, sb.data(), sb.size());
     int *psh = stage_put(hb, off, sh.data(), sh.size());
     int *psm = stage_put(hb, off, sm.data(), sm.size());
+    int *pss_start = stage_put(hb, off, ss.data(), ss.size());
     MSL_HIP_TRY(hipMemcpyAsync(R.d_stage, R.h_stage, off, hipMemcpyHostToDevice, st));
     auto dev = [&](const void *hp) { return R.d_stage + ((const char *)hp - hb); };
     const SBlur *gb = (const SBlur *)dev(pb);
     const SHeat *gh = (const SHeat *)dev(ph);
     const SMap *gm = (const SMap *)dev(pm);
+    const SSum *gss = (const SSum *)dev(pss);
     const SFinal *gf = (const SFinal *)dev(pf);
     const int *gsb = (const int *)dev(psb);
     const int *gsh = (const int *)dev(psh);
     const int *gsm = (const int *)dev(psm);
+    const int *gss_start = (const int *)dev(pss_start);
 
     // ---- launches
     const int cus = ctx().num_cus;
     auto persistent = [&](int n, int per_cu) { return std::max(1, std::min((n + 3) / 4, cus * per_cu)); };
-    if (items[0]) {
-        serve_expand_kernel<<<(items[0] + 255) / 256, 256, 0, st>>>(gsb, (int)db.size(), items[0], R.d_item_desc[0],
-                                                                    R.d_ctr);
-        serve_blur_kernel<<<persistent(items[0], 3), 256, 0, st>>>(gb, R.d_item_desc[0], items[0], R.d_pb, R.d_ctr);
-    }
-    if (items[1]) {
-        serve_expand_kernel<<<(items[1] + 255) / 256, 256, 0, st>>>(gsh, (int)dh.size(), items[1], R.d_item_desc[1],
-                                                                    R.d_ctr + 1);
-        serve_heat_kernel<<<persistent(items[1], 3), 256, 0, st>>>(gh, R.d_item_desc[1], items[1], R.d_ph,
-                                                                   R.d_ctr + 1);
-    }
+    // (map+fold jobs run before the stencil steps: they are single-pass and
+    // must not wait behind several steps of the stencil jobs)
     if (items[2]) {
         serve_expand_kernel<<<(items[2] + 255) / 256, 256, 0, st>>>(gsm, (int)dm.size(), items[2], R.d_item_desc[2],
-                                                                    R.d_ctr + 2);
+                                                                    R.d_ctr + 2 * kMaxSub, 1);
         serve_map_kernel<<<persistent(items[2], 4), 256, 0, st>>>(gm, R.d_item_desc[2], items[2], R.d_pm);
     }
+    if (items[0])
+        serve_expand_kernel<<<(items[0] + 255) / 256, 256, 0, st>>>(gsb, (int)db.size(), items[0], R.d_item_desc[0],
+                                                                    R.d_ctr, nsub_max[0]);
+    if (items[1])
+        serve_expand_kernel<<<(items[1] + 255) / 256, 256, 0, st>>>(gsh, (int)dh.size(), items[1], R.d_item_desc[1],
+                                                                    R.d_ctr + kMaxSub, nsub_max[1]);
+    for (int k = 0; k < std::max(nsub_max[0], nsub_max[1]); k++) {
+        if (k < nsub_max[0])
+            serve_blur_kernel<<<persistent(items[0], 3), 256, 0, st>>>(gb, R.d_item_desc[0], items[0], R.d_ctr + k,
+                                                                       k);
+        if (k < nsub_max[1])
+            serve_heat_kernel<<<persistent(items[1], 3), 256, 0, st>>>(gh, R.d_item_desc[1], items[1],
+                                                                       R.d_ctr + kMaxSub + k, k);
+    }
+    if (items[3]) {
+        serve_expand_kernel<<<(items[3] + 255) / 256, 256, 0, st>>>(gss_start, (int)ds.size(), items[3],
+                                                                    R.d_item_desc[3], R.d_ctr + 3 * kMaxSub, 1);
+        serve_sum_kernel<<<persistent(items[3], 4), 256, 0, st>>>(gss, R.d_item_desc[3], items[3], R.d_ps);
+    }
     if (!fin.empty()) {
-        serve_final_kernel<<<((int)fin.size() + 3) / 4, 256, 0, st>>>(gf, (int)fin.size(), R.d_pb, R.d_ph, R.d_pm,
+        serve_final_kernel<<<((int)fin.size() + 3) / 4, 256, 0, st>>>(gf, (int)fin.size(), R.d_ps, R.d_pm,
                                                                       R.d_rf, R.d_ru);
         MSL_HIP_TRY(hipMemcpyAsync(R.h_rf, R.d_rf, sizeof(double) * fin.size(), hipMemcpyDeviceToHost, st));
         MSL_HIP_TRY(hipMemcpyAsync(R.h_ru, R.d_ru, sizeof(uint32_t) * fin.size(), hipMemcpyDeviceToHost, st));
@@ -680,7 +774,7 @@ static int run_round(Server &S, int *did_work) {
     auto advance = [](std::vector<SJob *> &v) {
         size_t w = 0;
         for (SJob *J : v) {
-            J->steps_done++;
+            J->steps_done += J->nsub;
             if (J->steps_done < J->spec.steps) v[w++] = J;
         }
         v.resize(w);

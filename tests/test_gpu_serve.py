@@ -125,6 +125,23 @@ def test_alone_equals_mix():
         assert alone[0]["sum"] == m["sum"] and alone[0]["sum_u32"] == m["sum_u32"]
 
 
+def test_light_and_heavy_load_tilings_agree():
+    """A job served alone runs at light load (short segments, several steps
+    per round); in a mix whose items fill the GPU it runs with the default
+    segments, one step per round.  Final state and sum are bit-identical
+    either way, and match the oracle."""
+    note("server light vs heavy load tiling")
+    blur = dict(kind=JOB_BLUR2D, d0=1000, d1=1100, r=3, steps=12, seed=21, nv=0.5)
+    heavy = [dict(kind=JOB_HEAT3D, d0=256, d1=256, d2=256, steps=3, seed=22 + k) for k in range(2)]
+    alone, _ = serve_all([blur])
+    mix, _ = serve_all(heavy + [blur], budget=2 << 30)
+    np.testing.assert_array_equal(alone[0]["final"], mix[2]["final"])
+    assert alone[0]["sum"] == mix[2]["sum"]
+    check_job(blur, alone[0])
+    for d, res in zip(heavy, mix):
+        check_job(d, res)
+
+
 def test_tight_budget_admission():
     """A budget that holds only the largest job: jobs wait for memory, the
     results do not change."""
@@ -162,7 +179,7 @@ def test_run_trace_and_plain_loop_agree():
         if d["kind"] == JOB_MAPFOLD_U32:
             assert r["sum_u32"] == b["sum_u32"] == m["sum_u32"]
         else:
-            assert r["sum"] == m["sum"]  # same engine, same tiling
+            assert r["sum"] == m["sum"]  # same fold partition (fixed chunks of the final state)
             assert abs(r["sum"] - b["sum"]) <= 2 * FOLD_REL_TOL * max(abs(b["sum"]), 1e-30)
 
 
