@@ -19,6 +19,11 @@ static thread_local std::string g_last_error;
 static Context g_ctx;
 static std::mutex g_init_mutex;
 static void *g_ws = nullptr;
+// Pinned, device-accessible host buffer for small device-to-host copies: a
+// kernel writes the bytes there, which avoids the copy engine's latency
+// (a scalar fold result otherwise costs about as much as the fold).
+constexpr size_t kSmallCopyBytes = 64 << 10;
+static char *g_small = nullptr;
 static size_t g_ws_bytes = 0;
 static std::vector<void *> g_ws_retired;
 
@@ -56,6 +61,10 @@ int tune_int(const char *name, int dflt) {
     }
     cache[name] = v;
     return v;
+}
+
+__global__ void small_copy_kernel(char *__restrict__ dst, const char *__restrict__ src, size_t bytes) {
+    for (size_t i = threadIdx.x; i < bytes; i += blockDim.x) dst[i] = src[i];
 }
 
 int check_launch(const char *what) {
@@ -111,6 +120,7 @@ int msl_init(int device) {
     MSL_HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     MSL_HIP_TRY(hipEventCreate(&g_ctx.t0));
     MSL_HIP_TRY(hipEventCreate(&g_ctx.t1));
+    MSL_HIP_TRY(hipHostMalloc((void **)&g_small, kSmallCopyBytes, hipHostMallocDefault));
     g_ctx.ready = true;
     return MSL_OK;
 }
@@ -126,6 +136,10 @@ int msl_finalize(void) {
     }
     for (void *p : g_ws_retired) hipFree(p);
     g_ws_retired.clear();
+    if (g_small) {
+        (void)hipHostFree(g_small);
+        g_small = nullptr;
+    }
     hipEventDestroy(g_ctx.t0);
     hipEventDestroy(g_ctx.t1);
     hipStreamDestroy(g_ctx.stream);
@@ -197,6 +211,14 @@ int msl_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes) {
     if (rc) return rc;
     if (bytes == 0) return MSL_OK;
     MSL_REQUIRE(h_dst && d_src, "msl_memcpy_d2h: null pointer");
+    if (bytes <= kSmallCopyBytes && g_small && !g_ctx.capturing) {
+        small_copy_kernel<<<1, 256, 0, g_ctx.stream>>>(g_small, (const char *)d_src, bytes);
+        rc = check_launch("msl_memcpy_d2h");
+        if (rc) return rc;
+        MSL_HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        std::memcpy(h_dst, g_small, bytes);
+        return MSL_OK;
+    }
     MSL_HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
     MSL_HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     return MSL_OK;
