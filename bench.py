@@ -431,8 +431,16 @@ def bench_heat(hip, call, DeviceArray, extra):
     gbs = 8 * n * iters / t / 1e9
     out = {"workload": "DC<float> 512^3 heat7 (sum of 6 neighbours / 6, nv=0), 200 iterations, "
                        "fused max|delta| residual every 10, hipGraph loop",
-           "ms": round(t * 1e3, 3), "GBs": round(gbs, 1), "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 1),
+           "ms": round(t * 1e3, 3), "GBs": round(gbs, 1), "pct_hbm_peak_effective": round(100 * gbs / HBM_PEAK_GBS, 1),
+           "plan_generations_per_launch": hip.heat_iterate_plan(nx, ny, nz, iters, every),
            "residuals": [float(x) for x in res]}
+    tr = _profile_json("traffic_heat7_iterate200_512.json")
+    if tr:
+        hb = tr["hbm_bytes_per_call"] / t / 1e9
+        out.update({"achieved_hbm_GBs": round(hb, 1), "pct_hbm_peak_achieved": round(100 * hb / HBM_PEAK_GBS, 1)})
+    va = _profile_json("valu_heat7_iterate200_512.json")
+    if va:
+        out["valu_frac"] = round(va["valu_wave_instructions_per_call"] / VALU_ISSUE_PER_S / t, 4)
     extra["heat_final"] = (b if inb.value else a).to_host()
     log(f"[bench] heat {out['GBs']} GB/s")
     return out
@@ -477,6 +485,9 @@ def bench_batch(hip, extra):
            "persistent_ms": round(persistent_ms, 3), "loop_ms": round(loop_ms, 3),
            "speedup_vs_loop": round(loop_ms / persistent_ms, 2),
            "persistent_GBs": round(nbytes / (persistent_ms * 1e-3) / 1e9, 1),
+           "achieved_hbm_GBs": (round(BATCH_ITERS * _profile_json("traffic_batch_4096grids.json")["hbm_bytes_per_launch"]
+                                      / (persistent_ms * 1e-3) / 1e9, 1)
+                                if _profile_json("traffic_batch_4096grids.json") else None),
            "loop_GBs": round(nbytes / (loop_ms * 1e-3) / 1e9, 1),
            "pct_hbm_peak": round(100 * nbytes / (persistent_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
            "launches_loop": int(BATCH_GRIDS * (BATCH_ITERS + 1)), "launches_persistent_graph": BATCH_ITERS + 1,
