@@ -449,8 +449,10 @@ def bench_heat(hip, call, DeviceArray, extra):
 def bench_batch(hip, extra):
     """4096 independent grids (long side log-uniform in [32,1024], aspect up
     to 4:1, r uniform in 1..8), 10 mean-blur steps each plus a per-grid
-    fold(sum): persistent launches in one graph vs the plain loop of one
-    launch per grid per step and one fold per grid."""
+    fold(sum): launch rounds in one graph (two generations per item for
+    r <= 8) vs the plain loop of one launch per grid per step and one fold
+    per grid.  achieved_hbm_GBs: the HBM bytes rocprofv3 counted for one
+    run (profiles/traffic_batch_4096grids.json) over the measured time."""
     from muesli4_amd.batch import BatchArena
     from muesli4_amd.workloads import batch_shapes
 
@@ -478,6 +480,7 @@ def bench_batch(hip, extra):
     loop_ms = timed(lambda: bt.run_loop(BATCH_ITERS), 2)
     nbytes = bt.algorithmic_bytes(BATCH_ITERS)
     info = bt.info()
+    plan = bt.run_plan(BATCH_ITERS)
     tiles = {f"r{r}": [bt.tile(r, wc) for wc in range(4)] for r in range(1, 9)}
     out = {"workload": f"{BATCH_GRIDS} DM<float> grids, sides log-uniform [32,1024], aspect 1-4, r in 1..8, "
                        f"{BATCH_ITERS} mean-blur steps + per-grid compensated fold(sum)",
@@ -485,12 +488,13 @@ def bench_batch(hip, extra):
            "persistent_ms": round(persistent_ms, 3), "loop_ms": round(loop_ms, 3),
            "speedup_vs_loop": round(loop_ms / persistent_ms, 2),
            "persistent_GBs": round(nbytes / (persistent_ms * 1e-3) / 1e9, 1),
-           "achieved_hbm_GBs": (round(BATCH_ITERS * _profile_json("traffic_batch_4096grids.json")["hbm_bytes_per_launch"]
+           "achieved_hbm_GBs": (round(_profile_json("traffic_batch_4096grids.json")["hbm_bytes_per_run"]
                                       / (persistent_ms * 1e-3) / 1e9, 1)
                                 if _profile_json("traffic_batch_4096grids.json") else None),
            "loop_GBs": round(nbytes / (loop_ms * 1e-3) / 1e9, 1),
            "pct_hbm_peak": round(100 * nbytes / (persistent_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
-           "launches_loop": int(BATCH_GRIDS * (BATCH_ITERS + 1)), "launches_persistent_graph": BATCH_ITERS + 1,
+           "launches_loop": int(BATCH_GRIDS * (BATCH_ITERS + 1)), "launches_persistent_graph": plan["launches"],
+           "run_plan": plan,
            "autotune_s": round(tune_s, 2), "autotuned_classes": tuned, "launch": info,
            "tiles_cols_per_lane_seg_by_width_class": tiles}
     bt.reset()

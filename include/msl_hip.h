@@ -202,11 +202,17 @@ int msl_selftest_div_six(uint64_t *h_mismatches);
  * reads a and writes b, step 1 reads b and writes a, ...).  Every output
  * is bit-identical to msl_stencil2d_mean_f32 on that grid alone.
  *
- * msl_batch_run: each step is one persistent launch over a device work
- * queue of (grid, strip, segment) items; the steps and the per-grid fold
- * are captured once into a HIP graph (per iteration count and d_sums) and
- * replayed.  d_sums (nullable; ngrids doubles) receives the compensated
- * fold(sum) of each grid's final state (iters >= 1).
+ * msl_batch_run: the steps run in launch rounds, each a persistent launch
+ * over an atomic work queue of (grid, strip, segment) items; grids of
+ * radius <= 8 with 256-column strips advance two generations per item
+ * (pairs of steps, one pair split when the number of rounds would not
+ * have the parity of iters), the others one.  The rounds and the per-grid
+ * fold are captured once into a HIP graph (per iteration count and
+ * d_sums) and replayed.  d_sums (nullable; ngrids doubles) receives the
+ * compensated fold(sum) of each grid's final state (iters >= 1); the
+ * result is in b exactly when iters is odd (*h_result_in_b).
+ * MSL_BATCH_PAIRS=0 in the environment at msl_batch_create turns the
+ * two-generation items off; MSL_BATCH_PAIR_SEG sets their rows (128).
  * msl_batch_run_loop: the plain loop, one single-grid launch per grid per
  * step and one msl_fold_sum_f32 per grid (the comparison baseline).
  * Buffers must not be freed while the batch exists.                      */
@@ -238,6 +244,11 @@ int msl_batch_set_launch(void *batch, int blocks_per_cu, int queue_mode);
 int msl_batch_autotune(void *batch, const char *cache_path, int *h_entries);
 int msl_batch_info(const void *batch, int *h_items, int *h_blocks, int *h_blocks_per_cu, int *h_dynamic);
 int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b);
+/* The launch plan msl_batch_run would use (no device work): rounds,
+ * kernel launches in its graph, grids that run two generations per item
+ * and work items over all rounds; with_fold as d_sums != NULL.          */
+int msl_batch_run_plan(const void *batch, int iters, int with_fold, int *h_rounds, int *h_launches,
+                       int *h_pair_grids, int *h_items);
 int msl_batch_run_loop(void *batch, int iters, double *d_sums, int *h_result_in_b);
 
 /* ------------------------------------------------------------------ */

@@ -94,6 +94,12 @@ class BatchArena:
         call("msl_batch_info", self.handle, *[C.byref(x) for x in v])
         return {"items": v[0].value, "blocks": v[1].value, "blocks_per_cu": v[2].value, "dynamic": v[3].value}
 
+    def run_plan(self, iters: int, fold: bool = True) -> dict:
+        """Launch plan of run(iters): rounds, launches, two-generation grids, items."""
+        v = [C.c_int() for _ in range(4)]
+        call("msl_batch_run_plan", self.handle, iters, int(fold), *[C.byref(x) for x in v])
+        return {"rounds": v[0].value, "launches": v[1].value, "pair_grids": v[2].value, "items": v[3].value}
+
     def state(self, which: str = "a") -> np.ndarray:
         """Whole arena of state a or b on the host."""
         return (self.a if which == "a" else self.b).to_host()

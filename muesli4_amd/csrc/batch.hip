@@ -22,6 +22,18 @@
 // Either way consecutive items run on one XCD, so the halo rows shared by
 // neighbouring segments are served by one L2.
 //
+// Two generations per item.  In msl_batch_run, grids of radius up to
+// kBatchPairMaxR with 4-column strips run two generations per item
+// (strip2_march_dir: the generation-1 rows of a strip and its halo are kept
+// in the wave's LDS and the second generation is formed from them, so each
+// pair of generations reads and writes the grid once), in the launch plan
+// of the single-grid iterations (pairs, with one pair split so that the
+// number of launches keeps the parity of the step count); the other grids
+// run one generation per launch.  Each launch round is a pair kernel over
+// the pair items of that round and a step kernel over its single items.
+// Every cell keeps the operation order of one generation per launch, so the
+// states are bit-identical to the plain loop.
+//
 // Fold.  On the last step every lane keeps a TwoSum-compensated sum of the
 // values it stores; a wave combines its lanes in a fixed butterfly and
 // writes one (s, c) pair per item; a final kernel adds a grid's pairs in
@@ -116,6 +128,114 @@ __global__ __launch_bounds__(256) void batch_step_kernel(const BGrid *__restrict
     }
 }
 
+// ---- launch rounds of msl_batch_run ----------------------------------------
+// Radii up to 8 run two generations per item (the kernel needs 219 VGPRs,
+// two waves per SIMD, as for r <= 7; r <= 10 needs 256 and measured no
+// faster on the 4096-grid batch).  Items of 128 rows: the 4R extra input
+// rows of a two-generation march cost 44% at 64 rows and r = 7 (10 mean
+// blur steps of the 4096-grid batch: 5.69 ms at 64 rows, 5.50 at 96, 5.56
+// at 128 with r <= 7; 5.43 at 128 with r <= 8; 5.95 one generation per item).
+#ifndef MSL_BATCH_PAIR_MAX_R
+#define MSL_BATCH_PAIR_MAX_R 8
+#endif
+constexpr int kBatchPairMaxR = MSL_BATCH_PAIR_MAX_R;
+constexpr int kPairSeg = 128;  // rows per two-generation item
+constexpr int kPairLdsw = StripConsts<kBatchPairMaxR, 4>::LDSW;
+
+// One item of a launch round: grid g, strip at column xs, rows [y0, y1).
+// flags: kRUp march upwards, kRSrcB read state b (else a), kRFold fold the
+// stored values into partials[part].
+struct RItem {
+    int g, xs, y0, y1, flags, part;
+};
+constexpr int kRUp = 1, kRSrcB = 2, kRFold = 4;
+
+template <int R, bool kFold>
+__device__ __forceinline__ void pair_item(const float *src, float *dst, const BGrid &g, int xs, int y0, int y1,
+                                          float *ldsw, LaneSum *ls, int up) {
+    const MeanPolicy<R> pol = MeanPolicy<R>::make();
+    typedef float Row[StripConsts<R, 4>::LDSW];
+    Row *lds = reinterpret_cast<Row *>(ldsw);
+    if (g.kind == kKindAligned4) {
+        if (up)
+            strip2_march_dir<R, kAligned, true, true, kFold>(dst, src, g.rows, g.cols, g.nv, xs, y0, y1, pol, lds, ls);
+        else
+            strip2_march_dir<R, kAligned, false, true, kFold>(dst, src, g.rows, g.cols, g.nv, xs, y0, y1, pol, lds, ls);
+    } else {
+        if (up)
+            strip2_march_dir<R, kUnaligned, true, true, kFold>(dst, src, g.rows, g.cols, g.nv, xs, y0, y1, pol, lds, ls);
+        else
+            strip2_march_dir<R, kUnaligned, false, true, kFold>(dst, src, g.rows, g.cols, g.nv, xs, y0, y1, pol, lds,
+                                                                ls);
+    }
+}
+
+template <bool kFold, int R = 1>
+__device__ __forceinline__ void pair_item_r(int r, const float *src, float *dst, const BGrid &g, int xs, int y0, int y1,
+                                            float *ldsw, LaneSum *ls, int up) {
+    if constexpr (R < kBatchPairMaxR) {
+        if (r != R) return pair_item_r<kFold, R + 1>(r, src, dst, g, xs, y0, y1, ldsw, ls, up);
+    }
+    pair_item<R, kFold>(src, dst, g, xs, y0, y1, ldsw, ls, up);
+}
+
+// One launch of a round: every wave takes its first item statically and the
+// next ones from the round's atomic counter; kPair: two generations per item.
+template <bool kPair>
+__global__ __launch_bounds__(256) void batch_round_kernel(const BGrid *__restrict__ grids,
+                                                          const RItem *__restrict__ items, int nitems,
+                                                          float2 *__restrict__ partials, unsigned *__restrict__ ctr) {
+    __shared__ __attribute__((aligned(16))) float lds[4][kPair ? 4 * kPairLdsw : 2 * kBatchLdsw];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int stride = gridDim.x * 4;
+    for (int i = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave); i < nitems;) {
+        const RItem it = items[i];
+        const BGrid g = grids[it.g];
+        const float *src = (it.flags & kRSrcB) ? g.b : g.a;
+        float *dst = (it.flags & kRSrcB) ? g.a : g.b;
+        const int up = it.flags & kRUp;
+        LaneSum ls;
+        if (it.flags & kRFold) {
+            if constexpr (kPair)
+                pair_item_r<true>(g.r, src, dst, g, it.xs, it.y0, it.y1, lds[wave], &ls, up);
+            else
+                blur_item_r<true>(g.r, src, dst, g.rows, g.cols, g.nv, g.kind, it.xs, it.y0, it.y1, lds[wave], &ls, up);
+            wave_pair_reduce(ls);
+            if (lane == 0) partials[it.part] = make_float2(ls.s, ls.c);
+        } else {
+            if constexpr (kPair)
+                pair_item_r<false>(g.r, src, dst, g, it.xs, it.y0, it.y1, lds[wave], &ls, up);
+            else
+                blur_item_r<false>(g.r, src, dst, g.rows, g.cols, g.nv, g.kind, it.xs, it.y0, it.y1, lds[wave], &ls,
+                                   up);
+        }
+        unsigned nx = 0;
+        if (lane == 0) nx = atomicAdd(ctr, 1u);
+        i = stride + __builtin_amdgcn_readfirstlane((int)nx);
+    }
+}
+
+// One wave per grid: FP64 sum of the (s, c) pairs of its last round's items
+// (fold[g] = {first partial, count}), in item order.
+__global__ __launch_bounds__(256) void batch_round_final_kernel(const int2 *__restrict__ fold, int ngrids,
+                                                                const float2 *__restrict__ partials,
+                                                                double *__restrict__ sums) {
+    const int lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gi >= ngrids) return;
+    const int2 f = fold[gi];
+    double acc = 0.0;
+    for (int k = lane; k < f.y; k += 64) {
+        const float2 p = partials[f.x + k];
+        acc += (double)p.x;
+        acc += (double)p.y;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
+    if (lane == 0) sums[gi] = acc;
+}
+
 // One wave per grid: sum = sum over the grid's items, in item order, of
 // (double)s + (double)c; lanes take items l, l+64, ... and are combined in a
 // fixed butterfly.
@@ -153,6 +273,15 @@ struct Tile {
     int seg = 64;  // rows per item
 };
 
+// A captured msl_batch_run graph and the device tables it reads.
+struct RunGraph {
+    hipGraphExec_t exec = nullptr;
+    RItem *d_items = nullptr;
+    int2 *d_fold = nullptr;
+    float2 *d_partials = nullptr;
+    unsigned *d_ctr = nullptr;
+};
+
 struct Batch {
     std::vector<msl_batch_grid> grids;
     Tile table[kBatchMaxR + 1][kWidthClasses];
@@ -167,13 +296,23 @@ struct Batch {
     int ctr_cap = 0;
     int *d_bounds = nullptr;    // kXcd + 1 item-range bounds
     int nitems = 0, items_cap = 0;
-    std::map<std::pair<int, const void *>, hipGraphExec_t> graphs;
+    bool pairs = true;           // two generations per item in msl_batch_run
+    int max_pair_blocks_per_cu = 2;
+    int pair_seg = kPairSeg;
+    std::map<std::pair<int, const void *>, struct RunGraph> graphs;
 };
 
 static void drop_graphs(Batch &b) {
     if (b.graphs.empty()) return;
     (void)hipStreamSynchronize(ctx().stream);
-    for (auto &kv : b.graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : b.graphs) {
+        RunGraph &r = kv.second;
+        if (r.exec) (void)hipGraphExecDestroy(r.exec);
+        if (r.d_items) (void)hipFree(r.d_items);
+        if (r.d_fold) (void)hipFree(r.d_fold);
+        if (r.d_partials) (void)hipFree(r.d_partials);
+        if (r.d_ctr) (void)hipFree(r.d_ctr);
+    }
     b.graphs.clear();
 }
 
@@ -296,36 +435,6 @@ static int ensure_ctr(Batch &b, int n) {
     b.d_ctr = nullptr;
     MSL_HIP_TRY(hipMalloc(&b.d_ctr, sizeof(unsigned) * n));
     b.ctr_cap = n;
-    return MSL_OK;
-}
-
-// Enqueues `iters` persistent steps (the last with the fused fold when
-// d_sums is given) and the fold combine.
-static int enqueue_run(Batch &b, int iters, double *d_sums) {
-    const int ng = (int)b.grids.size();
-    const int blocks = launch_blocks(b);
-    hipStream_t s = ctx().stream;
-    if (b.dynamic) {
-        batch_zero_kernel<<<(iters * kXcd + 255) / 256, 256, 0, s>>>(b.d_ctr, iters * kXcd);
-        int rc = check_launch("msl_batch_run");
-        if (rc) return rc;
-    }
-    for (int it = 0; it < iters; it++) {
-        unsigned *ctr = b.dynamic ? b.d_ctr + it * kXcd : nullptr;
-        const bool fold = d_sums && it == iters - 1;
-        if (fold)
-            batch_step_kernel<true><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr,
-                                                           queue_bounds(b));
-        else
-            batch_step_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, b.d_items, b.nitems, it & 1, b.d_partials, ctr,
-                                                            queue_bounds(b));
-        int rc = check_launch("msl_batch_run");
-        if (rc) return rc;
-    }
-    if (d_sums) {
-        batch_fold_final_kernel<<<(ng + 3) / 4, 256, 0, s>>>(b.d_grids, ng, b.d_partials, d_sums);
-        return check_launch("msl_batch_run");
-    }
     return MSL_OK;
 }
 
@@ -492,6 +601,163 @@ static int autotune(Batch &b, int *entries) {
     return rebuild(b);
 }
 
+// Generations per launch of a grid that runs two per item: pairs, one split
+// in two when the number of launches would not have the parity of iters (the
+// result then ends in b exactly when iters is odd, as for the other grids).
+static std::vector<int> pair_plan(int iters) {
+    std::vector<int> plan;
+    for (int done = 0; done < iters; done += plan.back()) plan.push_back(std::min(2, iters - done));
+    if ((int)(plan.size() & 1) != (iters & 1)) {
+        for (size_t j = plan.size(); j-- > 0;)
+            if (plan[j] == 2) {
+                plan[j] = 1;
+                plan.insert(plan.begin() + j + 1, 1);
+                break;
+            }
+    }
+    return plan;
+}
+
+struct RoundRange {
+    int pair_off, pair_n, single_off, single_n;
+};
+
+// The launch rounds of msl_batch_run: per round the pair items and the
+// single items (tiles of the single-step table), each grid's items in its
+// round j reading state b when j is odd; the items of a grid's last round
+// fold into consecutive partials (fold[g] = {first, count}).
+struct RoundPlan {
+    std::vector<RItem> items;
+    std::vector<int2> fold;  // per grid {first partial, count}
+    std::vector<RoundRange> rounds;
+    int nparts = 0;
+    int pair_grids = 0;
+};
+
+static void plan_rounds(const Batch &b, int iters, bool fold, RoundPlan &rp) {
+    const int ng = (int)b.grids.size();
+    std::vector<std::vector<int>> plan(ng);
+    std::vector<int> kind(ng, kKindNarrow1), order;
+    for (int g = 0; g < ng; g++) {
+        const msl_batch_grid &G = b.grids[g];
+        if (G.rows == 0 || G.cols == 0) continue;
+        kind[g] = grid_kind(G, b.table[G.r][width_class(G.cols)].c);
+        const bool pair = b.pairs && G.r <= kBatchPairMaxR && kind[g] != kKindNarrow1;
+        plan[g] = pair ? pair_plan(iters) : std::vector<int>(iters, 1);
+        rp.pair_grids += pair ? 1 : 0;
+        order.push_back(g);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (b.grids[x].r != b.grids[y].r) return b.grids[x].r < b.grids[y].r;
+        return kind[x] < kind[y];
+    });
+    size_t nrounds = 0;
+    for (int g : order) nrounds = std::max(nrounds, plan[g].size());
+    std::vector<RItem> &items = rp.items;
+    std::vector<int2> &hfold = rp.fold;
+    hfold.assign(std::max(ng, 1), make_int2(0, 0));
+    int &nparts = rp.nparts;
+    std::vector<RoundRange> &rounds = rp.rounds;
+    for (size_t j = 0; j < nrounds; j++) {
+        RoundRange rr{};
+        for (int pass = 0; pass < 2; pass++) {  // pass 0: pair items, 1: single items
+            const int off = (int)items.size();
+            for (int g : order) {
+                if (j >= plan[g].size() || (plan[g][j] == 2) != (pass == 0)) continue;
+                const msl_batch_grid &G = b.grids[g];
+                const bool last = fold && j + 1 == plan[g].size();
+                int sw, seg;
+                if (pass == 0) {
+                    const int halo = (G.r + 3) / 4 * 4;
+                    sw = 256 - 2 * halo;
+                    seg = b.pair_seg;
+                } else {
+                    const Tile t = b.table[G.r][width_class(G.cols)];
+                    sw = 64 * (kind[g] == kKindNarrow1 ? 1 : 4);
+                    seg = std::max(1, t.seg);
+                }
+                if (last) hfold[g] = make_int2(nparts, 0);
+                for (int xs = 0; xs < G.cols; xs += sw)
+                    for (int y0 = 0; y0 < G.rows; y0 += seg) {
+                        int flags = ((j & 1) ? kRSrcB : 0) | (((y0 / seg) & 1) ? kRUp : 0);
+                        int part = 0;
+                        if (last) {
+                            flags |= kRFold;
+                            part = nparts++;
+                            hfold[g].y++;
+                        }
+                        items.push_back(RItem{g, xs, y0, std::min(y0 + seg, G.rows), flags, part});
+                    }
+            }
+            if (pass == 0) {
+                rr.pair_off = off;
+                rr.pair_n = (int)items.size() - off;
+            } else {
+                rr.single_off = off;
+                rr.single_n = (int)items.size() - off;
+            }
+        }
+        rounds.push_back(rr);
+    }
+}
+
+// Plans the rounds and uploads items and fold ranges into rg's buffers.
+static int build_rounds(Batch &b, int iters, bool fold, RunGraph &rg, std::vector<RoundRange> &rounds) {
+    RoundPlan rp;
+    plan_rounds(b, iters, fold, rp);
+    rounds = rp.rounds;
+    const std::vector<RItem> &items = rp.items;
+    const std::vector<int2> &hfold = rp.fold;
+    const int nparts = rp.nparts;
+    const size_t nrounds = rounds.size();
+    MSL_HIP_TRY(hipMalloc(&rg.d_items, sizeof(RItem) * std::max<size_t>(items.size(), 1)));
+    MSL_HIP_TRY(hipMalloc(&rg.d_fold, sizeof(int2) * hfold.size()));
+    MSL_HIP_TRY(hipMalloc(&rg.d_partials, sizeof(float2) * std::max(nparts, 1)));
+    MSL_HIP_TRY(hipMalloc(&rg.d_ctr, sizeof(unsigned) * std::max<size_t>(2 * nrounds, 1)));
+    if (!items.empty())
+        MSL_HIP_TRY(hipMemcpyAsync(rg.d_items, items.data(), sizeof(RItem) * items.size(), hipMemcpyHostToDevice,
+                                   ctx().stream));
+    MSL_HIP_TRY(hipMemcpyAsync(rg.d_fold, hfold.data(), sizeof(int2) * hfold.size(), hipMemcpyHostToDevice,
+                               ctx().stream));
+    MSL_HIP_TRY(hipStreamSynchronize(ctx().stream));
+    return MSL_OK;
+}
+
+// Enqueues the rounds (counters cleared by a kernel, never a memset node)
+// and the per-grid fold combine.
+static int enqueue_rounds(Batch &b, const std::vector<RoundRange> &rounds, double *d_sums, const RunGraph &rg) {
+    hipStream_t s = ctx().stream;
+    const int nc = (int)(2 * rounds.size());
+    if (nc > 0) {
+        batch_zero_kernel<<<(nc + 255) / 256, 256, 0, s>>>(rg.d_ctr, nc);
+        int rc = check_launch("msl_batch_run");
+        if (rc) return rc;
+    }
+    const int cus = ctx().num_cus;
+    for (size_t j = 0; j < rounds.size(); j++) {
+        const RoundRange &rr = rounds[j];
+        if (rr.pair_n) {
+            const int blocks = std::max(1, std::min((rr.pair_n + 3) / 4, cus * b.max_pair_blocks_per_cu));
+            batch_round_kernel<true><<<blocks, 256, 0, s>>>(b.d_grids, rg.d_items + rr.pair_off, rr.pair_n,
+                                                            rg.d_partials, rg.d_ctr + 2 * j);
+        }
+        if (rr.single_n) {
+            const int bpc = std::max(1, std::min(b.blocks_per_cu, b.max_blocks_per_cu));
+            const int blocks = std::max(1, std::min((rr.single_n + 3) / 4, cus * bpc));
+            batch_round_kernel<false><<<blocks, 256, 0, s>>>(b.d_grids, rg.d_items + rr.single_off, rr.single_n,
+                                                             rg.d_partials, rg.d_ctr + 2 * j + 1);
+        }
+        int rc = check_launch("msl_batch_run");
+        if (rc) return rc;
+    }
+    if (d_sums) {
+        const int ng = (int)b.grids.size();
+        batch_round_final_kernel<<<(ng + 3) / 4, 256, 0, s>>>(rg.d_fold, ng, rg.d_partials, d_sums);
+        return check_launch("msl_batch_run");
+    }
+    return MSL_OK;
+}
+
 static int check_batch(void *h, const char *what) {
     int rc = ensure_ready();
     if (rc) return rc;
@@ -530,6 +796,11 @@ int msl_batch_create(const msl_batch_grid *grids, int ngrids, void **h_batch) {
     b->blocks_per_cu = tune_int("MSL_BATCH_BLOCKS_PER_CU", 0);  // (cached per name: default applied below)
     if (b->blocks_per_cu <= 0) b->blocks_per_cu = b->max_blocks_per_cu;
     b->dynamic = tune_int("MSL_BATCH_DYNAMIC", 1);
+    if (const char *e = getenv("MSL_BATCH_PAIRS")) b->pairs = atoi(e) != 0;
+    if (const char *e = getenv("MSL_BATCH_PAIR_SEG")) b->pair_seg = std::max(8, atoi(e));
+    int pocc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pocc, batch_round_kernel<true>, 256, 0) == hipSuccess && pocc > 0)
+        b->max_pair_blocks_per_cu = pocc;
     for (int r = 0; r <= kBatchMaxR; r++)
         for (int wc = 0; wc < kWidthClasses; wc++) b->table[r][wc] = wc < 2 ? Tile{1, 64} : Tile{4, 64};
     if (ngrids > 0) {
@@ -627,6 +898,25 @@ int msl_batch_get_tile(const void *batch, int r, int width_class_idx, int *h_col
     return MSL_OK;
 }
 
+int msl_batch_run_plan(const void *batch, int iters, int with_fold, int *h_rounds, int *h_launches,
+                       int *h_pair_grids, int *h_items) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    MSL_REQUIRE(batch != nullptr, "msl_batch_run_plan: null batch");
+    MSL_REQUIRE(iters >= 0, "msl_batch_run_plan: negative iteration count");
+    const Batch *b = static_cast<const Batch *>(batch);
+    RoundPlan rp;
+    plan_rounds(*b, iters, with_fold != 0, rp);
+    int launches = rp.rounds.empty() ? 0 : 1;  // counter clear
+    for (const RoundRange &rr : rp.rounds) launches += (rr.pair_n > 0) + (rr.single_n > 0);
+    if (with_fold && iters > 0) launches++;
+    if (h_rounds) *h_rounds = (int)rp.rounds.size();
+    if (h_launches) *h_launches = launches;
+    if (h_pair_grids) *h_pair_grids = rp.pair_grids;
+    if (h_items) *h_items = (int)rp.items.size();
+    return MSL_OK;
+}
+
 int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b) {
     int rc = check_batch(batch, "msl_batch_run");
     if (rc) return rc;
@@ -635,24 +925,35 @@ int msl_batch_run(void *batch, int iters, double *d_sums, int *h_result_in_b) {
     Batch *b = static_cast<Batch *>(batch);
     if (h_result_in_b) *h_result_in_b = iters & 1;
     if (b->grids.empty()) return MSL_OK;
-    rc = ensure_ctr(*b, std::max(iters, 1) * kXcd);
-    if (rc) return rc;
     const auto key = std::make_pair(iters, (const void *)d_sums);
     auto it = b->graphs.find(key);
-    hipGraphExec_t exec = it == b->graphs.end() ? nullptr : it->second;
+    hipGraphExec_t exec = it == b->graphs.end() ? nullptr : it->second.exec;
     if (!exec) {
-        rc = msl_graph_begin();
-        if (rc) return rc;
-        int brc = enqueue_run(*b, iters, d_sums);
+        RunGraph rg;
+        std::vector<RoundRange> rounds;
+        rc = build_rounds(*b, iters, d_sums != nullptr, rg, rounds);
+        if (!rc) rc = msl_graph_begin();
+        if (rc) {
+            if (rg.d_items) (void)hipFree(rg.d_items);
+            if (rg.d_fold) (void)hipFree(rg.d_fold);
+            if (rg.d_partials) (void)hipFree(rg.d_partials);
+            if (rg.d_ctr) (void)hipFree(rg.d_ctr);
+            return rc;
+        }
+        int brc = enqueue_rounds(*b, rounds, d_sums, rg);
         void *g = nullptr;
         int erc = msl_graph_end(&g);
-        if (brc) {
+        if (brc || erc) {
             if (g) msl_graph_destroy(g);
-            return brc;
+            (void)hipFree(rg.d_items);
+            (void)hipFree(rg.d_fold);
+            (void)hipFree(rg.d_partials);
+            (void)hipFree(rg.d_ctr);
+            return brc ? brc : erc;
         }
-        if (erc) return erc;
-        exec = (hipGraphExec_t)g;
-        b->graphs[key] = exec;
+        rg.exec = (hipGraphExec_t)g;
+        exec = rg.exec;
+        b->graphs[key] = rg;
     }
     MSL_HIP_TRY(hipGraphLaunch(exec, ctx().stream));
     return MSL_OK;

@@ -423,11 +423,13 @@ __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, c
 // 5.1 -> 6.1 TB/s with 5 rows, r = 6 5.0 -> 5.8 with 2).
 constexpr int strip2_depth(int R, int d1) { return R == 5 ? 5 : (R == 6 ? 2 : d1); }
 
-template <int R, int MODE, bool UP, bool NT>
+// kSum: every stored value of generation 2 is also pushed into *sum (rows in
+// march order, the lane's columns ascending).
+template <int R, int MODE, bool UP, bool NT, bool kSum = false>
 __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const float *__restrict__ in, int rows,
                                                  int cols, float nv, int xs, int y0, int yend,
                                                  const MeanPolicy<R> &pol,
-                                                 float (*lds)[StripConsts<R, 4>::LDSW]) {
+                                                 float (*lds)[StripConsts<R, 4>::LDSW], LaneSum *sum = nullptr) {
     constexpr int C = 4;
     typedef StripConsts<R, C> K;
     constexpr int RP = K::RP;
@@ -527,8 +529,14 @@ __device__ __forceinline__ void strip2_march_dir(float *__restrict__ out, const 
                     acc = pol.acc(acc, ring2[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
                 o.v[c] = pol.finish(acc, 0.0f);
             }
-            if (store_lane && (UP ? y2 >= y0 : y2 < yend))
+            if (store_lane && (UP ? y2 >= y0 : y2 < yend)) {
                 write_vec<float, C, MODE, NT>(out + (size_t)y2 * cols, x1, cols, o);
+                if constexpr (kSum) {
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+                        if (x1 + c < cols) sum->push(o.v[c]);
+                }
+            }
         }
     }
 }

@@ -124,6 +124,7 @@ SIGNATURES = {
     "msl_batch_autotune": [p, C.c_char_p, PI],
     "msl_batch_info": [p, PI, PI, PI, PI],
     "msl_batch_run": [p, i32, p, PI],
+    "msl_batch_run_plan": [p, i32, i32, PI, PI, PI, PI],
     "msl_batch_run_loop": [p, i32, p, PI],
     "msl_server_create": [sz, i32, PP],
     "msl_server_destroy": [p],

@@ -58,6 +58,36 @@ def check_against_oracle(batch, iters, res_in_b, grids=None, oracle_init=None):
         assert abs(sums[g] - ref) <= FOLD_REL_TOL * max(abs(ref), 1e-30), (g, sums[g], ref)
 
 
+# Grids of radius up to 8 with 4-column strips run two generations per item
+# (launch plans 2+2, 2+2+1, 2+2+2+2+1+1 for 4, 5, 10 iterations); wide
+# grids of every radius 1..10, aligned and unaligned rows, both neutral
+# values, against the oracle.
+@pytest.mark.parametrize("iters", [4, 5, 10])
+def test_batch_pairs_match_oracle(iters):
+    note(f"batch pairs iters={iters}")
+    rows = np.array([300, 64, 200, 97, 130, 260, 45, 512, 33, 180, 70, 150, 90])
+    cols = np.array([256, 500, 513, 260, 777, 199, 1024, 300, 250, 384, 1001, 640, 333])
+    radii = np.array([1, 2, 3, 4, 5, 6, 7, 5, 7, 2, 8, 9, 10])
+    nv = np.where(np.arange(len(rows)) % 2 == 0, 0.25, 0.0).astype(np.float32)
+    b = BatchArena(rows, cols, radii, nv=nv, seed=9)
+    for wc in range(4):
+        for r in range(1, 11):
+            b.set_tile(r, wc, 4, 64)
+    plan = b.run_plan(iters)
+    assert plan["pair_grids"] == int(np.sum(radii <= 8))
+    # r = 9, 10 step once per round, so there are `iters` rounds, each with a
+    # one-generation launch, plus the pair launches, the counter clear and
+    # the fold combine
+    pair_launches = {4: 2, 5: 2, 10: 4}[iters]
+    assert plan["rounds"] == iters
+    assert plan["launches"] == 1 + iters + pair_launches + 1
+    res = b.run(iters)
+    hip.synchronize()
+    assert res == (iters & 1)
+    check_against_oracle(b, iters, res)
+    b.close()
+
+
 def small_batch(seed=5, n=40):
     rows, cols, radii = batch_shapes(n, seed, r_max=10, side_min=8, side_max=300)
     # edge shapes: a single row, a single column, cols < 4, radius larger
@@ -69,7 +99,7 @@ def small_batch(seed=5, n=40):
     return BatchArena(rows, cols, radii, nv=nv, seed=seed)
 
 
-@pytest.mark.parametrize("iters", [1, 2, 3])
+@pytest.mark.parametrize("iters", [1, 2, 3, 4, 5, 10])
 def test_batch_matches_oracle(iters):
     note(f"batch small iters={iters}")
     b = small_batch()
