@@ -29,10 +29,19 @@ constexpr int kStripW = StripConsts<1, 4>::SW;  // columns per wave strip
 // where four times as many strips keep more waves busy.
 // Radii 9 and 10 with 256-column strips: at most two waves per SIMD (the
 // register budget of the wave-uniform row addressing would allow three, but
-// a third wave per SIMD leaves a partial last round of the 16 waves per CU
-// that the segment shape aims at).
+// three waves per SIMD measured slower: spills, and a partial last round of
+// the waves strip_shape launches).
 template <int R, int C>
 constexpr int strip_max_waves_per_eu() { return (C == 4 && R >= 9 && R <= 10) ? 2 : 8; }
+
+// How the single-grid march ends its last ring period (strip_march_dir's
+// kTail): r = 10 runs 128-row segments on streamed grids (see strip_shape),
+// 148 input rows, one more than 7 ring periods of 21, so it ends with one
+// checked period (8192^2: 4.39 -> 4.69 TB/s against whole periods of 64-row
+// segments); every other radius keeps whole periods (r = 9 at 128 rows
+// marches 146 of 147 rows anyway and lost 2 % to the checked copy).
+template <int R, int C>
+constexpr int strip_tail_mode() { return (C == 4 && R == 10) ? 2 : 0; }
 
 template <typename Pol, int MODE, bool NT = true, int C = 4>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, strip_max_waves_per_eu<Pol::R, C>())))
@@ -57,7 +66,7 @@ void strip_stencil_kernel(typename Pol::T *__restrict__ out,
     // strip_shape), so radii up to 10 that otherwise prefetch one row ahead
     // prefetch three there (measured: r = 5, 6 and 8 gain 4-25 % on 4096^2).
     constexpr int DQ = (NT || StripConsts<R, C>::D > 1 || R > 10) ? 0 : 3;
-    strip_march<Pol, C, MODE, false, DQ, NT>(out, in, rows, cols, nv, sx * StripConsts<R, C>::SW, y0,
+    strip_march<Pol, C, MODE, false, DQ, NT, strip_tail_mode<R, C>()>(out, in, rows, cols, nv, sx * StripConsts<R, C>::SW, y0,
                                              min(y0 + seg, rows), pol, lds[wave], nullptr, sy & 1);
 }
 
@@ -115,7 +124,12 @@ inline StripShape strip_shape(int rows, int cols, int r, int seg, bool streamed)
     s.strips_x = (cols + sw - 1) / sw;
     if (seg <= 0) seg = tune_int("MSL_STRIP_SEG", 0);
     if (seg <= 0) {
-        const long target = cus * (s.c == 1 ? 32 : (streamed ? 16 : 8));
+        // radii 9 and 10 run at most two waves per SIMD with 256-column
+        // strips: one round of them (8 per CU) halves the 2R halo rows of
+        // each output row against two rounds (8192^2: r = 9 4.45 -> 4.68
+        // TB/s, r = 10 with the checked last period 4.39 -> 4.69)
+        const bool two_waves = s.c == 4 && r >= 9 && r <= 10;  // strip_max_waves_per_eu
+        const long target = cus * (s.c == 1 ? 32 : (streamed && !two_waves ? 16 : 8));
         const long segs = (target + s.strips_x - 1) / s.strips_x;
         seg = (int)((rows + segs - 1) / segs);
         const int seg_min = r <= 2 ? 4 : (r <= 6 ? 8 : 16);

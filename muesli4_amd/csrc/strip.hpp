@@ -283,7 +283,7 @@ constexpr int ring_slots(int PER, int D) {
 // register index is a compile-time constant.  Row t+D is requested when row t
 // is staged (DQ > 0 overrides the default prefetch depth).  With kSum, every
 // stored value is also pushed into *sum.
-template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP, bool NT, bool kTail>
+template <typename Pol, int C, int MODE, bool kSum, int DQ, bool UP, bool NT, int kTail>
 __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ out,
                                                 const typename Pol::T *__restrict__ in, int rows, int cols,
                                                 typename Pol::T nv, int xs, int y0, int yend, const Pol &pol,
@@ -325,64 +325,86 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
         qh[d] = load_vec<T, C, MODE>(in, ybase + ystep * d, hx, rows, cols);
     }
 
-    for (int tb = 0; tb < total; tb += UL) {
+    // One march step: stage input row t, request row t+D, form its row sums
+    // and, once 2R+1 rows are in the ring, finish output row t-R.  uu = t mod UL
+    // is a compile-time constant after unrolling, so every ring and queue
+    // index is static.
+    auto step = [&](const int t, const int uu) __attribute__((always_inline)) {
+        const int u = uu % P;  // ring slot of row t (tb is a multiple of P)
+        const int qs = uu % D;   // queue slot of row t (tb is a multiple of D)
+        T *buf = lds[t & 1];
+        typedef typename NativeVecN<T, C>::type V;
+        const int yy = ybase + ystep * t;
+        *reinterpret_cast<V *>(buf + RP + C * lane) =
+            to_native<T, C>(fix_vec<T, C, MODE>(qc[qs], yy, x0, rows, cols, nv));
+        *reinterpret_cast<V *>(buf + hslot) = to_native<T, C>(fix_vec<T, C, MODE>(qh[qs], yy, hx, rows, cols, nv));
+        // request row t+D into the slot just staged
+        qc[qs] = load_vec<T, C, MODE>(in, yy + ystep * D, x0, rows, cols);
+        qh[qs] = load_vec<T, C, MODE>(in, yy + ystep * D, hx, rows, cols);
+        wave_lds_sync();
+        T w[C * NQ];
 #pragma unroll
-        for (int uu = 0; uu < UL; uu++) {
-            const int t = tb + uu;
-            // kTail: the unrolled block may run past the march, and its last
-            // rows would only load, stage and sum rows that no output uses
-            // (up to a ring period per item: 6-30 % of the work of the short
-            // segments of the batch and serving engines).  Large single-grid
-            // marches lose more to the per-row exit than they save.
-            if constexpr (kTail)
-                if (t >= total) break;
-            const int u = uu % P;  // ring slot of row t (tb is a multiple of P)
-            const int qs = uu % D;   // queue slot of row t (tb is a multiple of D)
-            T *buf = lds[t & 1];
-            typedef typename NativeVecN<T, C>::type V;
-            const int yy = ybase + ystep * t;
-            *reinterpret_cast<V *>(buf + RP + C * lane) =
-                to_native<T, C>(fix_vec<T, C, MODE>(qc[qs], yy, x0, rows, cols, nv));
-            *reinterpret_cast<V *>(buf + hslot) = to_native<T, C>(fix_vec<T, C, MODE>(qh[qs], yy, hx, rows, cols, nv));
-            // request row t+D into the slot just staged
-            qc[qs] = load_vec<T, C, MODE>(in, yy + ystep * D, x0, rows, cols);
-            qh[qs] = load_vec<T, C, MODE>(in, yy + ystep * D, hx, rows, cols);
-            wave_lds_sync();
-            T w[C * NQ];
+        for (int q = 0; q < NQ; q++) {
+            const VecN<T, C> v = to_vecn<T, C>(*reinterpret_cast<const V *>(buf + C * lane + C * q));
 #pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const VecN<T, C> v = to_vecn<T, C>(*reinterpret_cast<const V *>(buf + C * lane + C * q));
+            for (int k = 0; k < C; k++) w[C * q + k] = v.v[k];
+        }
 #pragma unroll
-                for (int k = 0; k < C; k++) w[C * q + k] = v.v[k];
+        for (int c = 0; c < C; c++) ring[u].v[c] = pol.template rowsum<RP>(w, c);
+        if constexpr (Pol::kCenter) {
+#pragma unroll
+            for (int c = 0; c < C; c++) cring[u].v[c] = w[RP + c];
+        }
+        const int y = yy - ystep * R;  // output row completed by row t
+        if (t >= 2 * R && (UP ? y >= y0 : y < yend)) {
+            VecN<T, C> o;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                // rows y-R .. y+R in ascending order: the oldest ring entry
+                // first when marching down, the newest first when marching
+                // up (from the first row sum: see MeanPolicy::finish)
+                T acc = ring[UP ? u : (u + P - (PER - 1)) % P].v[c];
+#pragma unroll
+                for (int k = 1; k < PER; k++)
+                    acc = pol.acc(acc, ring[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
+                T centre = T(0);
+                if constexpr (Pol::kCenter) centre = cring[(u + P - R) % P].v[c];
+                o.v[c] = pol.finish(acc, centre);
             }
+            write_vec<T, C, MODE, NT>(out + (size_t)y * cols, x0, cols, o);
+            if constexpr (kSum) {
 #pragma unroll
-            for (int c = 0; c < C; c++) ring[u].v[c] = pol.template rowsum<RP>(w, c);
-            if constexpr (Pol::kCenter) {
-#pragma unroll
-                for (int c = 0; c < C; c++) cring[u].v[c] = w[RP + c];
+                for (int c = 0; c < C; c++)
+                    if (x0 + c < cols) sum->push(o.v[c]);
             }
-            const int y = yy - ystep * R;  // output row completed by row t
-            if (t >= 2 * R && (UP ? y >= y0 : y < yend)) {
-                VecN<T, C> o;
+        }
+    };
+    // kTail 0: whole ring periods; the steps past the march only load, stage
+    // and sum rows that no output uses.  kTail 1: every step checks the end
+    // (the batch and serving items, whose short segments would lose 6-30 % of
+    // their work to those steps).  kTail 2: whole periods without checks, then
+    // one checked period for the rest (long single-grid segments that do not
+    // fill whole periods).
+    int tb = 0;
+    if constexpr (kTail == 2) {
+        for (; tb + UL <= total; tb += UL) {
 #pragma unroll
-                for (int c = 0; c < C; c++) {
-                    // rows y-R .. y+R in ascending order: the oldest ring entry
-                    // first when marching down, the newest first when marching
-                    // up (from the first row sum: see MeanPolicy::finish)
-                    T acc = ring[UP ? u : (u + P - (PER - 1)) % P].v[c];
+            for (int uu = 0; uu < UL; uu++) step(tb + uu, uu);
+        }
+        if (tb < total) {
 #pragma unroll
-                    for (int k = 1; k < PER; k++)
-                        acc = pol.acc(acc, ring[UP ? (u + P - k) % P : (u + P - (PER - 1) + k) % P].v[c]);
-                    T centre = T(0);
-                    if constexpr (Pol::kCenter) centre = cring[(u + P - R) % P].v[c];
-                    o.v[c] = pol.finish(acc, centre);
-                }
-                write_vec<T, C, MODE, NT>(out + (size_t)y * cols, x0, cols, o);
-                if constexpr (kSum) {
+            for (int uu = 0; uu < UL; uu++) {
+                if (tb + uu >= total) break;
+                step(tb + uu, uu);
+            }
+        }
+    } else {
+        for (; tb < total; tb += UL) {
 #pragma unroll
-                    for (int c = 0; c < C; c++)
-                        if (x0 + c < cols) sum->push(o.v[c]);
-                }
+            for (int uu = 0; uu < UL; uu++) {
+                if constexpr (kTail == 1)
+                    if (tb + uu >= total) break;
+                step(tb + uu, uu);
             }
         }
     }
@@ -394,7 +416,7 @@ __device__ __forceinline__ void strip_march_dir(typename Pol::T *__restrict__ ou
 // adjacent segments makes both readers of the 2R halo rows between them
 // read those rows at the same time (both at the start or both at the end
 // of their marches), so the second read is served by L2.
-template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0, bool NT = true, bool kTail = false>
+template <typename Pol, int C, int MODE, bool kSum = false, int DQ = 0, bool NT = true, int kTail = 0>
 __device__ __forceinline__ void strip_march(typename Pol::T *__restrict__ out, const typename Pol::T *__restrict__ in,
                                             int rows, int cols, typename Pol::T nv, int xs, int y0, int yend,
                                             const Pol &pol, typename Pol::T (*lds)[StripConsts<Pol::R, C>::LDSW],
