@@ -80,7 +80,8 @@ __global__ __launch_bounds__(256) void strip2_mean_kernel(float *__restrict__ ou
     typedef StripConsts<R, 4> K;
     __shared__ __attribute__((aligned(16))) float lds[4][4][K::LDSW];
     const int wave = threadIdx.x >> 6;
-    const int item = (int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave;
+    // wave-uniform item: row indices and row bases of the march are scalar
+    const int item = __builtin_amdgcn_readfirstlane((int)xcd_tile(blockIdx.x, gridDim.x) * 4 + wave);
     if (item >= nitems) return;
     const int sx = item % strips_x;
     const int sy = item / strips_x;
