@@ -63,6 +63,12 @@ int msl_free(void *d_ptr);
 int msl_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
 int msl_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
 int msl_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+/* The library's pinned result slot (replaces the device-to-host copy of a
+ * fold result, src/ds.cpp:739-912): *h_slot is host memory that kernels may
+ * store to directly; its contents are valid after msl_synchronize() and are
+ * overwritten by the next call that uses the slot (small msl_memcpy_d2h,
+ * DS::fold).  64 KiB.                                                   */
+int msl_result_slot(void **h_slot, size_t *h_bytes);
 
 /* ------------------------------------------------------------------ */
 /* Fills (reference: DS::fill(const T&), src/ds.cpp:289-305; seeded

@@ -12,7 +12,8 @@
  * (one read, one write), 8 B/element for map, 12 for zip, 4 for fold.
  * Times are wall clock over back-to-back calls, except *_device: HIP events
  * on the library stream around one call (the fold's wall time also holds
- * the host round trip of its result).
+ * the host round trip of its result), and *_kernels: HIP events around the
+ * fold's two kernels alone.
  */
 #include <algorithm>
 #include <chrono>
@@ -164,6 +165,23 @@ int main(int argc, char **argv) {
         rate("dm_zip_functor_8192", 12 * cells, timed([&] { c.zip(a, b, add); }, 10));
         rate("dm_fold_functor_8192", 4 * cells, timed([&] { (void)a.fold(add); }, 10));
         rate("dm_fold_functor_8192_device", 4 * cells, device_timed([&] { (void)a.fold(add); }, 10));
+        {  // the two kernels DS::fold launches, with the result left in device memory
+            const float *d = a.getUserFunctionData();
+            const int blocks = detail::stream_grid((size_t)cells, 2);
+            float *part = nullptr;
+            int *has = nullptr;
+            msl_malloc((void **)&part, sizeof(float) * (blocks + 1));
+            msl_malloc((void **)&has, sizeof(int) * (blocks + 1));
+            rate("dm_fold_functor_8192_kernels", 4 * cells, device_timed([&] {
+                     detail::fold_stage1_kernel<float, Add><<<blocks, detail::kThreads, 0, detail::stream()>>>(
+                         d, (size_t)cells, part, has, add, detail::aligned16(d));
+                     detail::fold_stage2_kernel<float, Add><<<1, detail::kThreads, 0, detail::stream()>>>(
+                         part, has, blocks, part + blocks, has + blocks, add);
+                 }, 10));
+            msl_synchronize();
+            msl_free(part);
+            msl_free(has);
+        }
     }
     printf("{");
     for (size_t i = 0; i < out.size(); i++) printf("%s%s", i ? ", " : "", out[i].c_str());

@@ -224,6 +224,16 @@ int msl_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes) {
     return MSL_OK;
 }
 
+int msl_result_slot(void **h_slot, size_t *h_bytes) {
+    int rc = ensure_ready();
+    if (rc) return rc;
+    MSL_REQUIRE(h_slot != nullptr, "msl_result_slot: null out-pointer");
+    MSL_REQUIRE(g_small != nullptr, "msl_result_slot: no pinned slot");
+    *h_slot = g_small;
+    if (h_bytes) *h_bytes = kSmallCopyBytes;
+    return MSL_OK;
+}
+
 int msl_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes) {
     int rc = ensure_ready();
     if (rc) return rc;

@@ -86,6 +86,7 @@ SIGNATURES = {
     "msl_memcpy_h2d": [p, p, sz],
     "msl_memcpy_d2h": [p, p, sz],
     "msl_memcpy_d2d": [p, p, sz],
+    "msl_result_slot": [p, p],
     "msl_fill_f32": [p, sz, f32],
     "msl_fill_i32": [p, sz, C.c_int32],
     "msl_fill_uniform_f32": [p, sz, u64, f32],

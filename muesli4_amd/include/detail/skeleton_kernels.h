@@ -426,13 +426,13 @@ __device__ __forceinline__ void fold_block(T acc, bool has, F &f, T *out, int *o
     }
 }
 
-// Stage 1: kStreamUnroll 16-byte loads in flight per lane (4-byte types on
-// aligned buffers), then the element tail; one partial per block.  A thread
-// with at least one quad starts its accumulator from the first quad's first
-// element, so the unrolled loop folds without a "has a value" flag.
+// Stage 1 of one block: kStreamUnroll 16-byte loads in flight per lane
+// (4-byte types on aligned buffers), then the element tail; the block's
+// partial goes to *part / *part_has.  A thread with at least one quad starts
+// its accumulator from the first quad's first element, so the unrolled loop
+// folds without a "has a value" flag.
 template <typename T, typename F>
-__global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size_t n, T *partials, int *has, F f,
-                                                               int quad) {
+__device__ __forceinline__ void fold_stage1_block(const T *in, size_t n, T *part, int *part_has, F &f, int quad) {
     T acc{};
     bool h = false;
     size_t start = 0;
@@ -471,12 +471,14 @@ __global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size
         acc = h ? f.F::operator()(acc, in[i]) : in[i];
         h = true;
     }
-    fold_block(acc, h, f, partials + blockIdx.x, has + blockIdx.x);
+    fold_block(acc, h, f, part, part_has);
 }
 
+// Stage 2 of one block: the nparts partials in index order per thread, then
+// the block tree.
 template <typename T, typename F>
-__global__ __launch_bounds__(kThreads) void fold_stage2_kernel(const T *partials, const int *has, int nparts, T *result,
-                                                               int *result_has, F f) {
+__device__ __forceinline__ void fold_stage2_block(const T *partials, const int *has, int nparts, T *result,
+                                                  int *result_has, F &f) {
     T acc{};
     bool h = false;
     for (int i = threadIdx.x; i < nparts; i += kThreads) {
@@ -486,6 +488,19 @@ __global__ __launch_bounds__(kThreads) void fold_stage2_kernel(const T *partials
         }
     }
     fold_block(acc, h, f, result, result_has);
+}
+
+template <typename T, typename F>
+__global__ __launch_bounds__(kThreads) void fold_stage1_kernel(const T *in, size_t n, T *partials, int *has, F f,
+                                                               int quad) {
+    fold_stage1_block(in, n, partials + blockIdx.x, has + blockIdx.x, f, quad);
+}
+
+// *result may be host memory mapped for the device (the result slot).
+template <typename T, typename F>
+__global__ __launch_bounds__(kThreads) void fold_stage2_kernel(const T *partials, const int *has, int nparts, T *result,
+                                                               int *result_has, F f) {
+    fold_stage2_block(partials, has, nparts, result, result_has, f);
 }
 
 // ---- stencils ---------------------------------------------------------------------

@@ -140,22 +140,35 @@ public:
     // Device fold; the combination order is fixed (per-thread index order,
     // then a pairwise tree), so the result is deterministic for a device.
     // The flag of the reference (final fold on the CPU) is accepted and
-    // ignored: the final combine runs on the device, only T is copied back.
+    // ignored: the final combine runs on the device and stores the result
+    // straight into the library's pinned result slot (no copy launch).
+    // (Folding the partials in the last block of the first launch instead
+    // needs an agent-scope release per block, an L2 write-back each, which
+    // measured slower: 44.5 -> 53.5 us on 8192^2 floats.)
     template <typename FoldFunctor>
     T fold(FoldFunctor &f, bool /*final_fold_on_cpu*/ = true) {
         if (n == 0) throw detail::NotSameSizeException();
         updateDevice();
         const int blocks = detail::stream_grid(n, 2);
         ensureFoldScratch(blocks);
+        void *slot = nullptr;
+        size_t slot_bytes = 0;
+        detail::check(msl_result_slot(&slot, &slot_bytes), "DS::fold");
+        T *result = sizeof(T) <= slot_bytes ? static_cast<T *>(slot) : d_part + blocks;
         detail::fold_stage1_kernel<T, FoldFunctor>
             <<<blocks, detail::kThreads, 0, detail::stream()>>>(d_data, (size_t)n, d_part, d_has, f,
                                                                 detail::aligned16(d_data));
         detail::check_launch("DS::fold");
         detail::fold_stage2_kernel<T, FoldFunctor>
-            <<<1, detail::kThreads, 0, detail::stream()>>>(d_part, d_has, blocks, d_part + blocks, d_has + blocks, f);
+            <<<1, detail::kThreads, 0, detail::stream()>>>(d_part, d_has, blocks, result, d_has + blocks, f);
         detail::check_launch("DS::fold");
         T r{};
-        detail::check(msl_memcpy_d2h(&r, d_part + blocks, sizeof(T)), "DS::fold");
+        if (result == slot) {
+            detail::check(msl_synchronize(), "DS::fold");
+            std::memcpy(static_cast<void *>(&r), slot, sizeof(T));
+        } else {
+            detail::check(msl_memcpy_d2h(&r, result, sizeof(T)), "DS::fold");
+        }
         return r;
     }
     // Host fold in index order (src/ds.cpp:922-954 without its data race).

@@ -359,6 +359,23 @@ static void closed_forms() {
     mx.set(12000, 9.25f);
     Max fmax;
     check(mx.fold(fmax) == 9.25f, "DA fold max");
+    // many folds in a row on two large containers, each result read from the
+    // pinned result slot right after the previous one: integer sums exact
+    {
+        const int big = (1 << 24) + 13;
+        DA<int> u(big, 1), w(big, 0);
+        w.mapIndexInPlace(ip);  // i
+        w.mapInPlace(triple);   // 3i
+        const int expect_u = big;
+        // 3 * big * (big - 1) / 2 modulo 2^32, as the device's wrapping int adds give it
+        const int expect_w = (int)(unsigned)(3ull * (unsigned long long)big * (unsigned long long)(big - 1) / 2ull);
+        bool all = true;
+        for (int k = 0; k < 40; k++) {
+            all &= u.fold(add) == expect_u;
+            all &= w.fold(add) == expect_w;
+        }
+        check(all, "DA fold: 80 folds in a row, exact");
+    }
 
     // DM index skeletons (dm_test.cpp) and rotations
     const int rows = 37, cols = 53;
