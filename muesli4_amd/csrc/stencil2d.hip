@@ -193,14 +193,18 @@ int launch_mean2_r(float *out, const float *in, int rows, int cols, float nv, in
     const int strips_x = (cols + sw - 1) / sw;
     if (seg <= 0) seg = tune_int("MSL_STRIP2_SEG", 0);
     if (seg <= 0) {
-        // waves per CU on streamed grids: 24 for radii 2-4, 16 above
-        // (8192^2: r = 4 7.3 -> 8.2 TB/s with 24; r = 5 7.5 with 16, 6.4 with 24);
+        // waves per CU on streamed grids: 24 for r = 2, 8 above: one round
+        // of long segments halves the 4R warm-up rows of each segment
+        // (8192^2, 20 iterations, TB/s with 8 / 16 / 24 waves per CU:
+        // r = 2 8.70 / 8.68 / 8.79, r = 3 8.05 / 7.44 / 7.83, r = 5
+        // 7.05 / 6.06 / 5.86, r = 7 6.01 / 5.59 / 5.12);
         // MSL_STRIP2_WAVES overrides (tune_int caches one value per name, so
         // the radius-dependent default is applied here, not passed to it)
         int waves = tune_int("MSL_STRIP2_WAVES", 0);
-        if (waves <= 0) waves = nt ? (R <= 4 ? 24 : 16) : 8;
+        if (waves <= 0) waves = nt && R <= 2 ? 24 : 8;
+        // whole rounds: at most target waves, so no partial round trails
         const long target = cus * waves;
-        const long segs = (target + strips_x - 1) / strips_x;
+        const long segs = std::max(1L, target / strips_x);
         seg = (int)((rows + segs - 1) / segs);
         if (seg < 8 * R) seg = 8 * R;
     }
