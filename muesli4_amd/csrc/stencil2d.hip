@@ -131,7 +131,8 @@ inline StripShape strip_shape(int rows, int cols, int r, int seg, bool streamed)
         // TB/s, r = 10 with the checked last period 4.39 -> 4.69)
         const bool two_waves = s.c == 4 && r >= 9 && r <= 10;  // strip_max_waves_per_eu
         const long target = cus * (s.c == 1 ? 32 : (streamed && !two_waves ? 16 : 8));
-        const long segs = (target + s.strips_x - 1) / s.strips_x;
+        // at most target waves (whole rounds, no trailing partial one)
+        const long segs = std::max(1L, target / s.strips_x);
         seg = (int)((rows + segs - 1) / segs);
         const int seg_min = r <= 2 ? 4 : (r <= 6 ? 8 : 16);
         if (seg < seg_min) seg = seg_min;
