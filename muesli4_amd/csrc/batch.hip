@@ -623,9 +623,10 @@ struct RoundRange {
 };
 
 // The launch rounds of msl_batch_run: per round the pair items and the
-// single items (tiles of the single-step table), each grid's items in its
-// round j reading state b when j is odd; the items of a grid's last round
-// fold into consecutive partials (fold[g] = {first, count}).
+// single items (tiles of the single-step table), grids by descending radius,
+// each grid's items in its round j reading state b when j is odd; the items
+// of a grid's last round fold into consecutive partials (fold[g] = {first,
+// count}).
 struct RoundPlan {
     std::vector<RItem> items;
     std::vector<int2> fold;  // per grid {first partial, count}
@@ -647,8 +648,12 @@ static void plan_rounds(const Batch &b, int iters, bool fold, RoundPlan &rp) {
         rp.pair_grids += pair ? 1 : 0;
         order.push_back(g);
     }
+    // largest radius first: the costliest items start first and the cheap
+    // ones fill the end of each round (10 steps of the 4096-grid batch:
+    // 5.38 -> 5.04 ms against ascending radii); one radius and row mode at a
+    // time, so concurrent waves run the same specialisation
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        if (b.grids[x].r != b.grids[y].r) return b.grids[x].r < b.grids[y].r;
+        if (b.grids[x].r != b.grids[y].r) return b.grids[x].r > b.grids[y].r;
         return kind[x] < kind[y];
     });
     size_t nrounds = 0;
