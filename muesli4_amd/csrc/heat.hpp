@@ -135,9 +135,9 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
 }
 
 // Two generations per pass.  The wave's 64 lanes cover columns x0-2 ..
-// x0+61 and it keeps input rows y0-2 .. y0+kTY+1; generation 1 is computed
-// on rows y0-1 .. y0+kTY (exact on lanes 1 .. 62), generation 2 from it on
-// rows y0 .. y0+kTY-1, exact on lanes 2 .. 61 (columns x0 .. x0+59), which
+// x0+61 and it keeps input rows y0-2 .. y0+kT2Y+1; generation 1 is computed
+// on rows y0-1 .. y0+kT2Y (exact on lanes 1 .. 62), generation 2 from it on
+// rows y0 .. y0+kT2Y-1, exact on lanes 2 .. 61 (columns x0 .. x0+59), which
 // are stored: the two edge lanes on each side serve as x-halo, so no extra
 // halo loads are needed.  Along z generation 1 of plane z is formed first,
 // then generation 2 of plane z-1 from the ring of generation-1 planes.
@@ -146,12 +146,17 @@ __device__ __forceinline__ void heat7_march(float *__restrict__ out, const float
 // bit-identical to two single steps.  kResidual: max |generation 2 -
 // generation 1| into *dmax.
 constexpr int kT2X = kTX - 4;  // output columns of a two-generation wave tile
+// Output rows of a two-generation wave tile: 5 rows (90 VGPRs, five waves
+// per SIMD) beat 8 (124, four) although the y halo grows from 4/8 to 4/5
+// (512^3, 200 iterations: 8 rows 32.5 ms, 7 32.6, 6 31.8, 5 30.5, 4 35.1;
+// 320^3: 8 rows 7.13 ms, 5 6.90).
+constexpr int kT2Y = 5;
 
 template <bool kResidual, bool NT = true>
 __device__ __forceinline__ void heat7x2_march(float *__restrict__ out, const float *__restrict__ in, int nx, int ny,
                                               int nz, float nv, int x0, int y0, int z0, int z1, float *dmax) {
-    constexpr int NI = kTY + 4;  // input rows y0-2 .. y0+kTY+1
-    constexpr int N1 = kTY + 2;  // generation-1 rows y0-1 .. y0+kTY
+    constexpr int NI = kT2Y + 4;  // input rows y0-2 .. y0+kT2Y+1
+    constexpr int N1 = kT2Y + 2;  // generation-1 rows y0-1 .. y0+kT2Y
     const int lane = threadIdx.x & 63;
     const int x = x0 - 2 + lane;
     const int xc = min(max(x, 0), nx - 1);
@@ -226,7 +231,7 @@ __device__ __forceinline__ void heat7x2_march(float *__restrict__ out, const flo
             const int tb = (u + 3) & 3, tc = u & 3, ta = (u + 1) & 3;  // g1 slots of zo-1, zo, zo+1
             char *op = reinterpret_cast<char *>(out + (size_t)min(max(zo, 0), nz - 1) * plane);
 #pragma unroll
-            for (int j = 0; j < kTY; j++) {
+            for (int j = 0; j < kT2Y; j++) {
                 const float v = cell(g1[j], g1[j + 1], g1[j + 2], g1[j + 1], g1[j + 1], tb, tc, ta);
                 // output row y0+j is input row j+2: stored when in the cube
                 if (zo >= z0 && zo < z1 && lane >= 2 && lane <= 61 && ((iok >> (j + 2)) & 1u)) {

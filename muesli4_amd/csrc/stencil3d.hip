@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void heat7_kernel(float *__restrict__ out, con
     }
 }
 
-// Two generations per launch (heat7x2_march); tiles of kT2X x kTY columns.
+// Two generations per launch (heat7x2_march); tiles of kT2X x kT2Y columns.
 template <bool kResidual, bool NT>
 __global__ __launch_bounds__(256) void heat7x2_kernel(float *__restrict__ out, const float *__restrict__ in, int nx,
                                                       int ny, int nz, float nv, int zseg, int xtiles, int ytiles,
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void heat7x2_kernel(float *__restrict__ out, c
         const int yt = rest % ytiles;
         const int zs = rest / ytiles;
         const int z0 = zs * zseg;
-        heat7x2_march<kResidual, NT>(out, in, nx, ny, nz, nv, xt * kT2X, yt * kTY, z0, min(z0 + zseg, nz), &dmax);
+        heat7x2_march<kResidual, NT>(out, in, nx, ny, nz, nv, xt * kT2X, yt * kT2Y, z0, min(z0 + zseg, nz), &dmax);
     }
     if (kResidual) {
         __shared__ float wmax[4];
@@ -122,6 +122,7 @@ constexpr size_t kHeatPairMinBytes = (size_t)64 << 20;
 int heat_step2(float *out, const float *in, int nx, int ny, int nz, float nv, float *maxdiff) {
     HeatShape s = heat_shape(nx, ny, nz);
     s.xtiles = (nx + kT2X - 1) / kT2X;
+    s.ytiles = (ny + kT2Y - 1) / kT2Y;
     s.nitems = s.xtiles * s.ytiles * s.zsegs;
     const bool nt = stream_stores((size_t)nx * ny * nz * sizeof(float));
     const int blocks = (s.nitems + 3) / 4;
