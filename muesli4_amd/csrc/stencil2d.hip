@@ -265,11 +265,14 @@ int gol_step(int32_t *out, const int32_t *in, int rows, int cols) {
 // generations only cells at least s from every window edge are exact; the
 // block stores the central (64 - 2K) x (4*RPW - 2K) tile, exact after up to
 // K generations.  Integer rule: bit-identical to s single steps.
+// Whole-wave lane shifts with the vacated lane reading 0 (bound_ctrl): no
+// 'old' operand to materialise, so the compiler folds the shift into the
+// add that consumes it (v_add_f32_dpp / v_add_u32_dpp).
 __device__ __forceinline__ int32_t lane_from_left(int32_t v) {  // lane l gets lane l-1 (lane 0: 0)
-    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true);
 }
 __device__ __forceinline__ int32_t lane_from_right(int32_t v) {  // lane l gets lane l+1 (lane 63: 0)
-    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true);
 }
 
 template <int K, int RPW>
