@@ -358,23 +358,29 @@ __global__ __launch_bounds__(256) void mean3_multi_kernel(float *__restrict__ ou
         const int gy = gy0 + i;
         c[i] = (colok && (unsigned)gy < (unsigned)rows) ? in[(size_t)gy * cols + gx] : nv;
     }
+    // row sum of row i of the window (no leading +0: the column sum below
+    // is finished as if it started at +0, which makes a -0 row sum and a
+    // +0 one give the same result)
+    auto rowsum = [&](int i) {
+        const float l = __int_as_float(lane_from_left(__float_as_int(c[i])));
+        const float r = __int_as_float(lane_from_right(__float_as_int(c[i])));
+        return __fadd_rn(__fadd_rn(l, c[i]), r);
+    };
     for (int s = 0; s < steps; s++) {
         float rs[RPW + 2];  // row sums of rows gy0-1 .. gy0+RPW
-#pragma unroll
-        for (int i = 0; i < RPW; i++) {
-            const float l = __int_as_float(lane_from_left(__float_as_int(c[i])));
-            const float r = __int_as_float(lane_from_right(__float_as_int(c[i])));
-            // (no leading +0: the column sum below is finished as if it
-            // started at +0, which makes a -0 row sum and a +0 one give the
-            // same result)
-            rs[i + 1] = __fadd_rn(__fadd_rn(l, c[i]), r);
-        }
+        // the wave's edge rows first: their sums go to the neighbouring
+        // waves; the inner rows' sums follow the exchange, in the block of
+        // the column sums, where each lane shift folds into its add
+        rs[1] = rowsum(0);
+        rs[RPW] = rowsum(RPW - 1);
         edge[wave][0][lane] = rs[1];
         edge[wave][1][lane] = rs[RPW];
         __syncthreads();
         rs[0] = wave > 0 ? edge[wave - 1][1][lane] : 0.0f;
         rs[RPW + 1] = wave < 3 ? edge[wave + 1][0][lane] : 0.0f;
         __syncthreads();
+#pragma unroll
+        for (int i = 1; i < RPW - 1; i++) rs[i + 1] = rowsum(i);
 #pragma unroll
         for (int i = 0; i < RPW; i++) {
             // the column sum without its leading +0, which div_plus0 restores
