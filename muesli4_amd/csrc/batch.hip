@@ -529,7 +529,7 @@ static int objective(Batch &b, float *ms) {
     return MSL_OK;
 }
 
-// Coordinate descent on the whole batch, so every candidate is judged with
+// Coordinate descent on the whole batch, so every candidate is measured with
 // the device full of the other classes' work (a class timed alone under-fills
 // the device and favours short segments):
 //   1. launch shape: workgroups per CU (the two largest that fit) x queue;
