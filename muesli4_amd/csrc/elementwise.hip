@@ -14,7 +14,9 @@ typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
-constexpr int kUnroll = 4;
+constexpr int kUnroll = 4;      // 16-byte accesses in flight per lane, one input
+constexpr int kZipUnroll = 2;   // two inputs: the same four loads in flight
+                                // (8192^2 zip: 5.68 TB/s with 4 per input, 6.01 with 2)
 
 // Plain loads and non-temporal stores: measured fastest on MI355X for
 // streams (the non-temporal store keeps the output from evicting the input
@@ -67,15 +69,15 @@ __global__ __launch_bounds__(kThreads) void stream2_kernel(S *__restrict__ out, 
     V *out4 = reinterpret_cast<V *>(out);
     const size_t stride = (size_t)gridDim.x * kThreads;
     size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    for (; i + (kUnroll - 1) * stride < n4; i += kUnroll * stride) {
-        V va[kUnroll], vb[kUnroll];
+    for (; i + (kZipUnroll - 1) * stride < n4; i += kZipUnroll * stride) {
+        V va[kZipUnroll], vb[kZipUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
+        for (int u = 0; u < kZipUnroll; u++) {
             va[u] = ldg(a4 + i + u * stride);
             vb[u] = ldg(b4 + i + u * stride);
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) stg(out4 + i + u * stride, op.vec(va[u], vb[u]));
+        for (int u = 0; u < kZipUnroll; u++) stg(out4 + i + u * stride, op.vec(va[u], vb[u]));
     }
     for (; i < n4; i += stride) stg(out4 + i, op.vec(ldg(a4 + i), ldg(b4 + i)));
     size_t t = n4 * 4 + (size_t)blockIdx.x * kThreads + threadIdx.x;
